@@ -1,0 +1,470 @@
+// C ABI of libctlib_amd.so (declared in include/ctlib_abi.h).
+//
+// Replaces the reference's dispatch layer (ctlib.cpp:15-99) and its per-geometry
+// host wrappers (fan_ed_kernel.cu:1163-1382, fan_ea_kernel.cu:923-1107,
+// para_kernel.cu:685-849).  Differences by design:
+//   * options are parsed once into an immutable device-resident plan;
+//   * ctlib_run launches on the caller's stream, allocates nothing and never
+//     synchronises (the reference makes 6-12 .item() device syncs per call);
+//   * outputs are written exactly once (the reference zero-fills and then
+//     accumulates with global atomics);
+//   * invalid input returns a status code (the reference exit(0)s on an unknown
+//     scan_type, ctlib.cpp:26).
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <string>
+
+#include "../../include/ctlib_abi.h"
+#include "kernels.h"
+#include "plan.h"
+
+using namespace ctlib;
+
+struct ctlib_plan {
+    HostPlan host;
+    int device = 0;
+    void* dmem = nullptr;
+    DevPlan dev{};
+};
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& why) {
+    g_err = why;
+    return code;
+}
+
+int hip_fail(hipError_t e, const char* what) {
+    g_err = std::string(what) + ": " + hipGetErrorString(e);
+    return CTLIB_ERR_HIP;
+}
+
+// Restores the calling thread's current device on scope exit.
+struct DeviceGuard {
+    int prev = -1;
+    bool ok = false;
+    explicit DeviceGuard(int dev) {
+        if (hipGetDevice(&prev) != hipSuccess) return;
+        if (prev != dev && hipSetDevice(dev) != hipSuccess) return;
+        ok = true;
+    }
+    ~DeviceGuard() {
+        if (prev >= 0) (void)hipSetDevice(prev);
+    }
+};
+
+inline size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
+
+size_t dsize(int dtype) { return dtype == CTLIB_F64 ? sizeof(double) : sizeof(float); }
+
+void to_geom(const ctlib_geom* in, Geom* g) {
+    g->views = in->views;
+    g->dets = in->dets;
+    g->width = in->width;
+    g->height = in->height;
+    g->scan_type = in->scan_type;
+    g->dImg = in->dImg;
+    g->dDet = in->dDet;
+    g->Ang0 = in->Ang0;
+    g->dAng = in->dAng;
+    g->s2r = in->s2r;
+    g->d2r = in->d2r;
+    g->binshift = in->binshift;
+}
+
+// Scratch layout of every op (byte offsets into ws).
+struct WsLayout {
+    size_t ylines = 0, tables = 0, yt = 0, rows = 0, total = 0;
+    size_t n_ylines = 0, n_tables = 0, n_yt = 0, n_rows = 0;
+};
+
+WsLayout layout(const ctlib_plan* p, int op, int64_t B, int dtype) {
+    const DevPlan& P = p->dev;
+    const size_t es = dsize(dtype);
+    const size_t img = (size_t)B * P.H * P.W;
+    const size_t sino = (size_t)B * P.V * P.D;
+    WsLayout L;
+    switch (op) {
+    case CTLIB_OP_PRJ:
+    case CTLIB_OP_BPRJ_T:
+    case CTLIB_OP_BPRJ_T_TRUE:
+        L.n_ylines = P.ny ? img * es : 0;
+        break;
+    case CTLIB_OP_PRJ_T:
+    case CTLIB_OP_BPRJ:
+        L.n_tables = (size_t)B * P.V * (P.D + 2) * sizeof(double2);
+        L.n_yt = P.ny ? img * es : 0;
+        break;
+    case CTLIB_OP_FBP:
+        L.n_rows = sino * sizeof(double);
+        L.n_tables = (size_t)B * P.V * (P.D + 2) * sizeof(double2);
+        L.n_yt = P.ny ? img * es : 0;
+        break;
+    case CTLIB_OP_FBP_T:
+        L.n_ylines = P.ny ? img * es : 0;
+        L.n_rows = sino * es;
+        break;
+    default:
+        break;
+    }
+    size_t off = 0;
+    L.ylines = off; off += align_up(L.n_ylines);
+    L.tables = off; off += align_up(L.n_tables);
+    L.yt = off;     off += align_up(L.n_yt);
+    L.rows = off;   off += align_up(L.n_rows);
+    L.total = off;
+    return L;
+}
+
+FwdShape default_fwd_shape(const DevPlan& P, int mode, bool weighted) {
+    // Largest view group whose LDS footprint keeps >= 2 blocks per CU.
+    for (int G : {8, 4, 2}) {
+        if (weighted && G != 2) continue;
+        if (fwd_lds_bytes(P, mode, weighted, G) <= 72 * 1024) return FwdShape{G};
+    }
+    return FwdShape{2};
+}
+
+int check_run_args(const ctlib_plan* p, int op, int dtype, const void* in, void* out, int64_t B, void* ws,
+                   size_t ws_bytes) {
+    if (!p) return fail(CTLIB_ERR_INVALID_ARG, "plan is NULL");
+    if (op < CTLIB_OP_PRJ || op > CTLIB_OP_FBP_T) return fail(CTLIB_ERR_INVALID_ARG, "unknown op code");
+    if (dtype != CTLIB_F32 && dtype != CTLIB_F64) return fail(CTLIB_ERR_UNSUPPORTED_DTYPE, "dtype must be f32 or f64");
+    if (B < 0 || B > (1 << 24)) return fail(CTLIB_ERR_INVALID_ARG, "batch out of range");
+    if (B > 0 && (!in || !out)) return fail(CTLIB_ERR_INVALID_ARG, "in/out must be device pointers");
+    const size_t need = layout(p, op, B, dtype).total;
+    if (B > 0 && need > 0 && (!ws || ws_bytes < need))
+        return fail(CTLIB_ERR_WORKSPACE, "workspace smaller than ctlib_workspace_bytes()");
+    return CTLIB_OK;
+}
+
+// Sinogram-output ops: projection and backprojection_t.
+hipError_t run_to_sino(const ctlib_plan* p, int dtype, const void* in, void* out, int B, bool weighted,
+                       const double* post, const WsLayout& L, char* ws, hipStream_t s) {
+    const DevPlan& P = p->dev;
+    hipError_t e = hipSuccess;
+    if (P.nx) e = launch_fwd(P, dtype, in, out, MODE_X, B, weighted, post, default_fwd_shape(P, MODE_X, weighted), s);
+    if (e != hipSuccess) return e;
+    if (P.ny) {
+        void* yl = ws + L.ylines;
+        e = launch_to_ylines(in, yl, dtype, B, P.H, P.W, s);
+        if (e != hipSuccess) return e;
+        e = launch_fwd(P, dtype, yl, out, MODE_Y, B, weighted, post, default_fwd_shape(P, MODE_Y, weighted), s);
+    }
+    return e;
+}
+
+// Image-output ops from prepared bin tables: projection_t and backprojection.
+hipError_t run_to_image(const ctlib_plan* p, int dtype, const double2* tabs, void* out, int B, bool weighted,
+                        double wscale, const WsLayout& L, char* ws, hipStream_t s) {
+    const DevPlan& P = p->dev;
+    void* yt = ws + L.yt;
+    hipError_t e = hipSuccess;
+    if (P.nx) e = launch_adj(P, dtype, tabs, out, nullptr, MODE_X, B, weighted, wscale, AdjShape{0}, s);
+    if (e != hipSuccess) return e;
+    if (P.ny) e = launch_adj(P, dtype, tabs, nullptr, yt, MODE_Y, B, weighted, wscale, AdjShape{0}, s);
+    if (e != hipSuccess) return e;
+    if (P.ny) e = launch_combine(out, yt, dtype, B, P.H, P.W, P.nx > 0, true, s);
+    return e;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* ctlib_last_error(void) { return g_err.c_str(); }
+
+const char* ctlib_status_string(int status) {
+    switch (status) {
+    case CTLIB_OK: return "ok";
+    case CTLIB_ERR_INVALID_ARG: return "invalid argument";
+    case CTLIB_ERR_BAD_SCAN_TYPE: return "bad scan_type";
+    case CTLIB_ERR_SHAPE: return "shape mismatch";
+    case CTLIB_ERR_HIP: return "HIP runtime error";
+    case CTLIB_ERR_UNSUPPORTED_DTYPE: return "unsupported dtype";
+    case CTLIB_ERR_WORKSPACE: return "workspace too small";
+    case CTLIB_ERR_GEOMETRY: return "unsupported geometry";
+    default: return "unknown status";
+    }
+}
+
+int ctlib_parse_options(const double* opts, int n, ctlib_geom* out) {
+    if (!out) return fail(CTLIB_ERR_INVALID_ARG, "output geometry is NULL");
+    Geom g{};
+    std::string why;
+    int rc = parse_options(opts, n, &g, &why);
+    if (rc) return fail(rc, why);
+    out->views = g.views;
+    out->dets = g.dets;
+    out->width = g.width;
+    out->height = g.height;
+    out->scan_type = g.scan_type;
+    out->dImg = g.dImg;
+    out->dDet = g.dDet;
+    out->Ang0 = g.Ang0;
+    out->dAng = g.dAng;
+    out->s2r = g.s2r;
+    out->d2r = g.d2r;
+    out->binshift = g.binshift;
+    return CTLIB_OK;
+}
+
+int ctlib_plan_create(const ctlib_geom* geom, int device, ctlib_plan** out) {
+    if (!geom || !out) return fail(CTLIB_ERR_INVALID_ARG, "NULL argument");
+    *out = nullptr;
+    Geom g{};
+    to_geom(geom, &g);
+    if (g.views < 1 || g.dets < 1 || g.width < 1 || g.height < 1)
+        return fail(CTLIB_ERR_INVALID_ARG, "sizes must be positive");
+    if (g.dets > 2046 || g.width > 2048 || g.height > 2048)
+        return fail(CTLIB_ERR_GEOMETRY, "sizes above dets 2046 / width, height 2048 are not supported");
+    ctlib_plan* p = new (std::nothrow) ctlib_plan();
+    if (!p) return fail(CTLIB_ERR_INVALID_ARG, "out of host memory");
+    std::string why;
+    int rc = build_plan(g, &p->host, &why);
+    if (rc) {
+        delete p;
+        return fail(rc, why);
+    }
+    p->device = device;
+    const HostPlan& H = p->host;
+    const int V = g.views, D = g.dets;
+    // One device allocation holds every table.
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        size_t o = off;
+        off += align_up(bytes);
+        return o;
+    };
+    const size_t o_vp = take(sizeof(ViewParams) * V);
+    const size_t o_edges = take(sizeof(double) * V * (D + 1));
+    const size_t o_pp = take(sizeof(double) * V * D);
+    const size_t o_pb = take(sizeof(double) * V * D);
+    const size_t o_lines = take(sizeof(LineCoef) * H.lines.size());
+    const size_t o_xv = take(sizeof(int) * (H.xviews.size() + 1));
+    const size_t o_yv = take(sizeof(int) * (H.yviews.size() + 1));
+    const size_t o_taps = take(sizeof(double) * H.ramp_taps.size());
+    const size_t o_pw = take(sizeof(double) * H.ramp_preweight.size());
+    const size_t total = off;
+
+    DeviceGuard guard(device);
+    if (!guard.ok) {
+        delete p;
+        return fail(CTLIB_ERR_HIP, "cannot select device " + std::to_string(device));
+    }
+    hipError_t e = hipMalloc(&p->dmem, total);
+    if (e != hipSuccess) {
+        delete p;
+        return hip_fail(e, "hipMalloc(plan)");
+    }
+    char* base = static_cast<char*>(p->dmem);
+    struct Copy { size_t off; const void* src; size_t bytes; };
+    const Copy copies[] = {
+        {o_vp, H.vp.data(), sizeof(ViewParams) * V},
+        {o_edges, H.edges.data(), sizeof(double) * V * (D + 1)},
+        {o_pp, H.post_prj.data(), sizeof(double) * V * D},
+        {o_pb, H.post_bprj_t.data(), sizeof(double) * V * D},
+        {o_lines, H.lines.data(), sizeof(LineCoef) * H.lines.size()},
+        {o_xv, H.xviews.data(), sizeof(int) * H.xviews.size()},
+        {o_yv, H.yviews.data(), sizeof(int) * H.yviews.size()},
+        {o_taps, H.ramp_taps.data(), sizeof(double) * H.ramp_taps.size()},
+        {o_pw, H.ramp_preweight.data(), sizeof(double) * H.ramp_preweight.size()},
+    };
+    for (const Copy& c : copies) {
+        if (!c.bytes) continue;
+        e = hipMemcpy(base + c.off, c.src, c.bytes, hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            (void)hipFree(p->dmem);
+            delete p;
+            return hip_fail(e, "hipMemcpy(plan)");
+        }
+    }
+    DevPlan& P = p->dev;
+    P.vp = reinterpret_cast<const ViewParams*>(base + o_vp);
+    P.edges = reinterpret_cast<const double*>(base + o_edges);
+    P.post_prj = reinterpret_cast<const double*>(base + o_pp);
+    P.post_bt = reinterpret_cast<const double*>(base + o_pb);
+    P.lines = reinterpret_cast<const LineCoef*>(base + o_lines);
+    P.xviews = reinterpret_cast<const int*>(base + o_xv);
+    P.yviews = reinterpret_cast<const int*>(base + o_yv);
+    P.taps = reinterpret_cast<const double*>(base + o_taps);
+    P.preweight = reinterpret_cast<const double*>(base + o_pw);
+    P.nx = (int)H.xviews.size();
+    P.ny = (int)H.yviews.size();
+    P.V = V;
+    P.D = D;
+    P.W = g.width;
+    P.H = g.height;
+    P.max_lines = H.max_lines;
+    P.scan_type = g.scan_type;
+    P.dAng = g.dAng;
+    *out = p;
+    return CTLIB_OK;
+}
+
+void ctlib_plan_destroy(ctlib_plan* p) {
+    if (!p) return;
+    if (p->dmem) {
+        DeviceGuard guard(p->device);
+        (void)hipFree(p->dmem);
+    }
+    delete p;
+}
+
+size_t ctlib_workspace_bytes(const ctlib_plan* p, int op, int64_t batch, int dtype) {
+    if (!p || batch <= 0) return 0;
+    return layout(p, op, batch, dtype).total;
+}
+
+int ctlib_plan_geom(const ctlib_plan* p, ctlib_geom* out) {
+    if (!p || !out) return fail(CTLIB_ERR_INVALID_ARG, "NULL argument");
+    const Geom& g = p->host.g;
+    out->views = g.views;
+    out->dets = g.dets;
+    out->width = g.width;
+    out->height = g.height;
+    out->scan_type = g.scan_type;
+    out->dImg = g.dImg;
+    out->dDet = g.dDet;
+    out->Ang0 = g.Ang0;
+    out->dAng = g.dAng;
+    out->s2r = g.s2r;
+    out->d2r = g.d2r;
+    out->binshift = g.binshift;
+    return CTLIB_OK;
+}
+
+int ctlib_plan_view_info(const ctlib_plan* p, int view, int* xmode, int* rev) {
+    if (!p || !xmode || !rev) return fail(CTLIB_ERR_INVALID_ARG, "NULL argument");
+    if (view < 0 || view >= p->host.g.views) return fail(CTLIB_ERR_INVALID_ARG, "view out of range");
+    *xmode = p->host.vp[view].xmode;
+    *rev = p->host.vp[view].rev;
+    return CTLIB_OK;
+}
+
+int ctlib_plan_export(const ctlib_geom* geom, int table, void* dst, size_t* bytes) {
+    if (!geom || !bytes) return fail(CTLIB_ERR_INVALID_ARG, "NULL argument");
+    Geom g{};
+    to_geom(geom, &g);
+    HostPlan H;
+    std::string why;
+    int rc = build_plan(g, &H, &why);
+    if (rc) return fail(rc, why);
+    const void* src = nullptr;
+    size_t n = 0;
+    switch (table) {
+    case 0: src = H.vp.data(); n = H.vp.size() * sizeof(ViewParams); break;
+    case 1: src = H.edges.data(); n = H.edges.size() * sizeof(double); break;
+    case 2: src = H.post_prj.data(); n = H.post_prj.size() * sizeof(double); break;
+    case 3: src = H.post_bprj_t.data(); n = H.post_bprj_t.size() * sizeof(double); break;
+    case 4: src = H.lines.data(); n = H.lines.size() * sizeof(LineCoef); break;
+    case 5: src = H.xviews.data(); n = H.xviews.size() * sizeof(int); break;
+    case 6: src = H.yviews.data(); n = H.yviews.size() * sizeof(int); break;
+    case 7: src = H.ramp_taps.data(); n = H.ramp_taps.size() * sizeof(double); break;
+    case 8: src = H.ramp_preweight.data(); n = H.ramp_preweight.size() * sizeof(double); break;
+    default: return fail(CTLIB_ERR_INVALID_ARG, "unknown table");
+    }
+    if (dst) {
+        if (*bytes < n) return fail(CTLIB_ERR_INVALID_ARG, "destination too small");
+        if (n) std::memcpy(dst, src, n);
+    }
+    *bytes = n;
+    return CTLIB_OK;
+}
+
+int ctlib_run(const ctlib_plan* p, int op, int dtype, const void* in, void* out, int64_t batch, void* ws,
+              size_t ws_bytes, void* stream) {
+    int rc = check_run_args(p, op, dtype, in, out, batch, ws, ws_bytes);
+    if (rc) return rc;
+    if (batch == 0) return CTLIB_OK;
+    DeviceGuard guard(p->device);
+    if (!guard.ok) return fail(CTLIB_ERR_HIP, "cannot select the plan's device");
+    const DevPlan& P = p->dev;
+    const int B = (int)batch;
+    const int dt = dtype == CTLIB_F64 ? DT_F64 : DT_F32;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const WsLayout L = layout(p, op, batch, dtype);
+    char* w = static_cast<char*>(ws);
+    double2* tabs = reinterpret_cast<double2*>(w + L.tables);
+    hipError_t e = hipSuccess;
+    switch (op) {
+    case CTLIB_OP_PRJ:
+        e = run_to_sino(p, dt, in, out, B, false, P.post_prj, L, w, s);
+        break;
+    case CTLIB_OP_BPRJ_T:
+        e = run_to_sino(p, dt, in, out, B, true, P.scan_type == SCAN_FAN_EA ? P.post_bt : nullptr, L, w, s);
+        break;
+    case CTLIB_OP_BPRJ_T_TRUE:
+        e = run_to_sino(p, dt, in, out, B, true, nullptr, L, w, s);
+        break;
+    case CTLIB_OP_PRJ_T:
+        e = launch_bin_tables(P, dt, in, nullptr, P.post_prj, 1.0, tabs, B, s);
+        if (e == hipSuccess) e = run_to_image(p, dt, tabs, out, B, false, 1.0, L, w, s);
+        break;
+    case CTLIB_OP_BPRJ:
+        e = launch_bin_tables(P, dt, in, nullptr, nullptr, 1.0, tabs, B, s);
+        if (e == hipSuccess) e = run_to_image(p, dt, tabs, out, B, true, 1.0, L, w, s);
+        break;
+    case CTLIB_OP_FBP: {
+        // pre-weight + RL ramp (Toeplitz GEMM) -> weighted backprojection * dAng/2 (fan_ed_kernel.cu:1349-1380)
+        double* rows = reinterpret_cast<double*>(w + L.rows);
+        e = launch_ramp(P, dt, in, DT_F64, rows, (long)B * P.V, P.preweight, nullptr, 1.0, s);
+        if (e == hipSuccess) e = launch_bin_tables(P, DT_F64, rows, nullptr, nullptr, 1.0, tabs, B, s);
+        if (e == hipSuccess) e = run_to_image(p, dt, tabs, out, B, true, P.dAng / 2, L, w, s);
+        break;
+    }
+    case CTLIB_OP_FBP_T: {
+        // adjoint of fbp: exact backprojection adjoint * dAng/2, then ramp, then pre-weight
+        void* rows = w + L.rows;
+        e = run_to_sino(p, dt, in, rows, B, true, nullptr, L, w, s);
+        if (e == hipSuccess) e = launch_ramp(P, dt, rows, dt, out, (long)B * P.V, nullptr, P.preweight, P.dAng / 2, s);
+        break;
+    }
+    default:
+        return fail(CTLIB_ERR_INVALID_ARG, "unknown op code");
+    }
+    if (e != hipSuccess) return hip_fail(e, "kernel launch");
+    return CTLIB_OK;
+}
+
+size_t ctlib_residual_workspace_bytes(const ctlib_plan* p, int64_t batch, int dtype) {
+    if (!p || batch <= 0) return 0;
+    const WsLayout a = layout(p, CTLIB_OP_PRJ, batch, dtype);
+    const WsLayout b = layout(p, CTLIB_OP_PRJ_T, batch, dtype);
+    const size_t sino = align_up((size_t)batch * p->dev.V * p->dev.D * dsize(dtype));
+    return (a.total > b.total ? a.total : b.total) + sino;
+}
+
+int ctlib_run_residual(const ctlib_plan* p, int dtype, const void* x, const void* m, void* y, void* grad,
+                       int64_t batch, void* ws, size_t ws_bytes, void* stream) {
+    int rc = check_run_args(p, CTLIB_OP_PRJ_T, dtype, x, grad, batch, ws, ws_bytes);
+    if (rc) return rc;
+    if (batch == 0) return CTLIB_OK;
+    if (!m) return fail(CTLIB_ERR_INVALID_ARG, "measurement pointer is NULL");
+    if (ws_bytes < ctlib_residual_workspace_bytes(p, batch, dtype))
+        return fail(CTLIB_ERR_WORKSPACE, "workspace smaller than ctlib_residual_workspace_bytes()");
+    DeviceGuard guard(p->device);
+    if (!guard.ok) return fail(CTLIB_ERR_HIP, "cannot select the plan's device");
+    const DevPlan& P = p->dev;
+    const int B = (int)batch;
+    const int dt = dtype == CTLIB_F64 ? DT_F64 : DT_F32;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    char* w = static_cast<char*>(ws);
+    const WsLayout la = layout(p, CTLIB_OP_PRJ, batch, dtype);
+    const WsLayout lb = layout(p, CTLIB_OP_PRJ_T, batch, dtype);
+    const size_t base = la.total > lb.total ? la.total : lb.total;
+    void* ybuf = y ? y : static_cast<void*>(w + base);
+    hipError_t e = run_to_sino(p, dt, x, ybuf, B, false, P.post_prj, la, w, s);
+    double2* tabs = reinterpret_cast<double2*>(w + lb.tables);
+    if (e == hipSuccess) e = launch_bin_tables(P, dt, ybuf, m, P.post_prj, 1.0, tabs, B, s);
+    if (e == hipSuccess) e = run_to_image(p, dt, tabs, grad, B, false, 1.0, lb, w, s);
+    if (e != hipSuccess) return hip_fail(e, "kernel launch");
+    return CTLIB_OK;
+}
+
+}  // extern "C"

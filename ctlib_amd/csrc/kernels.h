@@ -1,0 +1,62 @@
+// Device-side plan view and kernel launchers shared by the C ABI (abi.cpp) and
+// the HIP kernels (dd_kernels.hip, ramp_mfma.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "plan.h"
+
+namespace ctlib {
+
+// Device pointers into one plan allocation (all read-only after upload).
+struct DevPlan {
+    const ViewParams* vp;     // [V]
+    const double* edges;      // [V][D+1]
+    const double* post_prj;   // [V][D]
+    const double* post_bt;    // [V][D]
+    const LineCoef* lines;    // [V][max_lines]
+    const int* xviews;        // [nx]
+    const int* yviews;        // [ny]
+    const double* taps;       // [2D-1]
+    const double* preweight;  // [D]
+    int nx, ny;
+    int V, D, W, H, max_lines;
+    int scan_type;
+    double dAng;
+};
+
+enum { DT_F32 = 0, DT_F64 = 1 };
+enum { MODE_X = 0, MODE_Y = 1 };
+
+// Launch-shape choices the autotuner may change (see tune.cpp).
+struct FwdShape { int G; };       // views per block of the line->bin kernel
+struct AdjShape { int lpw; };     // lines per wave of the bin->pixel kernel (0 = default)
+
+// x [B][H][W] -> xT [B][W][H] with xT[b][c][i] = x[b][H-1-i][c]  (Y-mode lines, bottom to top)
+hipError_t launch_to_ylines(const void* x, void* xT, int dtype, int B, int H, int W, hipStream_t s);
+
+// Line -> bin operator (projection; weighted = backprojection_t) for the views of one sweep axis.
+// `src` holds the lines of that axis ([B][H][W] for X, the transposed copy for Y).
+hipError_t launch_fwd(const DevPlan& P, int dtype, const void* src, void* y, int mode, int B, bool weighted,
+                      const double* post, FwdShape shape, hipStream_t s);
+size_t fwd_lds_bytes(const DevPlan& P, int mode, bool weighted, int G);
+
+// Per (b, view) cumulative-integral tables {C_j, g_j}, j = 0..D+1, of g_k = (y - m)[o(k)] * pre[k] * scale.
+hipError_t launch_bin_tables(const DevPlan& P, int in_dtype, const void* y, const void* m, const double* pre,
+                             double scale, double2* tabs, int B, hipStream_t s);
+
+// Bin -> pixel operator (projection_t; weighted = backprojection, pixel weight * wscale) for one axis.
+// X lines are written to outX [B][H][W]; Y lines to outYT [B][W][H] (bottom-to-top per column).
+hipError_t launch_adj(const DevPlan& P, int dtype, const double2* tabs, void* outX, void* outYT, int mode, int B,
+                      bool weighted, double wscale, AdjShape shape, hipStream_t s);
+
+// out[b][r][c] = (hasX ? out[b][r][c] : 0) + (hasY ? yt[b][c][H-1-r] : 0)
+hipError_t launch_combine(void* out, const void* yt, int dtype, int B, int H, int W, bool hasX, bool hasY,
+                          hipStream_t s);
+
+// Ramp filtering of `rows` sinogram rows as a dense Toeplitz GEMM on FP64 MFMA:
+//   out[r][j] = scale * post[j] * sum_m (pre[m] * in[r][m]) * h(m - j)     (pre/post may be null = 1)
+// with h the RL taps of the plan.  in/out dtypes are DT_F32 or DT_F64.
+hipError_t launch_ramp(const DevPlan& P, int in_dtype, const void* in, int out_dtype, void* out, long rows,
+                       const double* pre, const double* post, double scale, hipStream_t s);
+
+}  // namespace ctlib

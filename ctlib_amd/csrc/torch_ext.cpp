@@ -1,0 +1,240 @@
+// PyTorch-ROCm extension `ctlib_amd._C`: the reference's Python surface
+// (ctlib.cpp:106-114) on top of the C ABI of libctlib_amd.so.
+//
+// Each op keeps the reference's signature f(Tensor, Tensor options) -> Tensor,
+// tensor layout (B x 1 x H x W images, B x 1 x V x D sinograms, channel 0 read,
+// output channel 1) and dtype set (float32, float64).  Differences:
+//   * options may live on the host (no device sync) or on the device (one
+//     D2H copy), in either float dtype; the reference needs device options of
+//     the data dtype (fan_ed_kernel.cu:1188-1191);
+//   * shapes are checked against the options (the reference reads out of
+//     bounds on a mismatch) and bad scan_types raise instead of exit(0);
+//   * kernels run on the current HIP stream under a device guard (the
+//     reference uses the legacy default stream and leaves cudaSetDevice set).
+#include <torch/extension.h>
+
+#include <c10/hip/HIPGuard.h>
+#include <c10/hip/HIPStream.h>
+
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/ctlib_abi.h"
+
+namespace {
+
+struct PlanDeleter {
+    void operator()(ctlib_plan* p) const { ctlib_plan_destroy(p); }
+};
+using PlanPtr = std::shared_ptr<ctlib_plan>;
+
+struct Cached {
+    PlanPtr plan;
+    ctlib_geom geom;
+};
+
+std::mutex g_mu;
+std::map<std::string, Cached> g_cache;
+constexpr size_t kMaxPlans = 64;
+
+[[noreturn]] void raise_status(int rc, const char* what) {
+    std::string msg = std::string(what) + ": " + ctlib_status_string(rc) + ": " + ctlib_last_error();
+    if (rc == CTLIB_ERR_INVALID_ARG || rc == CTLIB_ERR_BAD_SCAN_TYPE || rc == CTLIB_ERR_SHAPE ||
+        rc == CTLIB_ERR_GEOMETRY)
+        throw pybind11::value_error(msg);
+    if (rc == CTLIB_ERR_UNSUPPORTED_DTYPE) throw pybind11::type_error(msg);
+    throw std::runtime_error(msg);
+}
+
+std::vector<double> options_values(const torch::Tensor& options) {
+    TORCH_CHECK(options.dim() == 1, "options must be a 1-D tensor");
+    TORCH_CHECK(options.scalar_type() == torch::kFloat32 || options.scalar_type() == torch::kFloat64,
+                "options must be float32 or float64");
+    torch::Tensor h = options.detach().to(torch::kCPU, torch::kFloat64).contiguous();
+    const double* p = h.data_ptr<double>();
+    return std::vector<double>(p, p + h.numel());
+}
+
+Cached get_plan(const torch::Tensor& options, int device) {
+    std::vector<double> vals = options_values(options);
+    ctlib_geom g{};
+    int rc = ctlib_parse_options(vals.data(), (int)vals.size(), &g);
+    if (rc) raise_status(rc, "options");
+    std::string key(reinterpret_cast<const char*>(&g), sizeof(g));
+    key.append(reinterpret_cast<const char*>(&device), sizeof(device));
+    std::lock_guard<std::mutex> lock(g_mu);
+    auto it = g_cache.find(key);
+    if (it != g_cache.end()) return it->second;
+    ctlib_plan* raw = nullptr;
+    rc = ctlib_plan_create(&g, device, &raw);
+    if (rc) raise_status(rc, "plan");
+    if (g_cache.size() >= kMaxPlans) g_cache.clear();
+    Cached c{PlanPtr(raw, PlanDeleter()), g};
+    g_cache.emplace(key, c);
+    return c;
+}
+
+int dtype_code(const torch::Tensor& t) {
+    if (t.scalar_type() == torch::kFloat32) return CTLIB_F32;
+    if (t.scalar_type() == torch::kFloat64) return CTLIB_F64;
+    throw pybind11::type_error(std::string("ctlib ops support float32 and float64 tensors, got ") +
+                               std::string(c10::toString(t.scalar_type())));
+}
+
+// CHECK_INPUT of ctlib.cpp:7-9, plus the shape checks the reference omits.
+torch::Tensor check_input(const torch::Tensor& x, const char* name) {
+    TORCH_CHECK(x.is_cuda(), name, " must be a GPU (HIP) tensor");
+    TORCH_CHECK(x.is_contiguous(), name, " must be contiguous");
+    TORCH_CHECK(x.dim() == 4, name, " must be 4-D (B x 1 x rows x cols), got ", x.dim(), "-D");
+    if (x.size(1) != 1) return x.select(1, 0).unsqueeze(1).contiguous();  // reference reads channel 0
+    return x;
+}
+
+enum Kind { IMG_TO_SINO, SINO_TO_IMG };
+
+torch::Tensor run_op(const torch::Tensor& in_, const torch::Tensor& options, int op, Kind kind, const char* name) {
+    torch::Tensor in = check_input(in_, name);
+    const int dt = dtype_code(in);
+    const int device = in.get_device();
+    c10::hip::HIPGuard guard((c10::DeviceIndex)device);
+    Cached c = get_plan(options, device);
+    const ctlib_geom& g = c.geom;
+    const int64_t B = in.size(0);
+    if (kind == IMG_TO_SINO) {
+        if (in.size(2) != g.height || in.size(3) != g.width)
+            throw pybind11::value_error(std::string(name) + " is " + std::to_string(in.size(2)) + "x" +
+                                        std::to_string(in.size(3)) + " but options give height x width = " +
+                                        std::to_string(g.height) + "x" + std::to_string(g.width));
+    } else {
+        if (in.size(2) != g.views || in.size(3) != g.dets)
+            throw pybind11::value_error(std::string(name) + " is " + std::to_string(in.size(2)) + "x" +
+                                        std::to_string(in.size(3)) + " but options give views x dets = " +
+                                        std::to_string(g.views) + "x" + std::to_string(g.dets));
+    }
+    torch::Tensor out = kind == IMG_TO_SINO ? torch::empty({B, 1, g.views, g.dets}, in.options())
+                                            : torch::empty({B, 1, g.height, g.width}, in.options());
+    if (B == 0) return out;
+    const size_t wsb = ctlib_workspace_bytes(c.plan.get(), op, B, dt);
+    torch::Tensor ws;
+    if (wsb) ws = torch::empty({(int64_t)wsb}, in.options().dtype(torch::kUInt8));
+    hipStream_t s = c10::hip::getCurrentHIPStream((c10::DeviceIndex)device).stream();
+    int rc = ctlib_run(c.plan.get(), op, dt, in.data_ptr(), out.data_ptr(), B, wsb ? ws.data_ptr() : nullptr, wsb,
+                       (void*)s);
+    if (rc) raise_status(rc, name);
+    return out;
+}
+
+// ctlib.cpp:56-69
+torch::Tensor projection(torch::Tensor image, torch::Tensor options) {
+    return run_op(image, options, CTLIB_OP_PRJ, IMG_TO_SINO, "image");
+}
+// ctlib.cpp:71-84
+torch::Tensor projection_t(torch::Tensor projection, torch::Tensor options) {
+    return run_op(projection, options, CTLIB_OP_PRJ_T, SINO_TO_IMG, "projection");
+}
+// ctlib.cpp:30-43
+torch::Tensor backprojection(torch::Tensor projection, torch::Tensor options) {
+    return run_op(projection, options, CTLIB_OP_BPRJ, SINO_TO_IMG, "projection");
+}
+// ctlib.cpp:15-28
+torch::Tensor backprojection_t(torch::Tensor image, torch::Tensor options) {
+    return run_op(image, options, CTLIB_OP_BPRJ_T, IMG_TO_SINO, "image");
+}
+// ctlib.cpp:86-99
+torch::Tensor fbp(torch::Tensor projection, torch::Tensor options) {
+    return run_op(projection, options, CTLIB_OP_FBP, SINO_TO_IMG, "projection");
+}
+// Exact adjoints used by autograd (no reference counterpart).
+torch::Tensor backprojection_t_true(torch::Tensor image, torch::Tensor options) {
+    return run_op(image, options, CTLIB_OP_BPRJ_T_TRUE, IMG_TO_SINO, "image");
+}
+torch::Tensor fbp_t(torch::Tensor image, torch::Tensor options) {
+    return run_op(image, options, CTLIB_OP_FBP_T, IMG_TO_SINO, "image");
+}
+
+// One least-squares step: returns (A x, A^T (A x - m)).
+std::vector<torch::Tensor> residual_step(torch::Tensor image, torch::Tensor measured, torch::Tensor options) {
+    torch::Tensor x = check_input(image, "image");
+    torch::Tensor m = check_input(measured, "measured");
+    const int dt = dtype_code(x);
+    TORCH_CHECK(m.scalar_type() == x.scalar_type(), "measured must have the image dtype");
+    TORCH_CHECK(m.get_device() == x.get_device(), "image and measured must be on the same device");
+    const int device = x.get_device();
+    c10::hip::HIPGuard guard((c10::DeviceIndex)device);
+    Cached c = get_plan(options, device);
+    const ctlib_geom& g = c.geom;
+    const int64_t B = x.size(0);
+    if (x.size(2) != g.height || x.size(3) != g.width)
+        throw pybind11::value_error("image shape does not match the options");
+    if (m.size(0) != B || m.size(2) != g.views || m.size(3) != g.dets)
+        throw pybind11::value_error("measured sinogram shape does not match the options / batch");
+    torch::Tensor y = torch::empty({B, 1, g.views, g.dets}, x.options());
+    torch::Tensor grad = torch::empty({B, 1, g.height, g.width}, x.options());
+    if (B == 0) return {y, grad};
+    const size_t wsb = ctlib_residual_workspace_bytes(c.plan.get(), B, dt);
+    torch::Tensor ws = torch::empty({(int64_t)wsb}, x.options().dtype(torch::kUInt8));
+    hipStream_t s = c10::hip::getCurrentHIPStream((c10::DeviceIndex)device).stream();
+    int rc = ctlib_run_residual(c.plan.get(), dt, x.data_ptr(), m.data_ptr(), y.data_ptr(), grad.data_ptr(), B,
+                                ws.data_ptr(), wsb, (void*)s);
+    if (rc) raise_status(rc, "residual_step");
+    return {y, grad};
+}
+
+// Geometry of an options vector as the library parses it (for host-side code and tests).
+pybind11::dict parse_options(torch::Tensor options) {
+    std::vector<double> vals = options_values(options);
+    ctlib_geom g{};
+    int rc = ctlib_parse_options(vals.data(), (int)vals.size(), &g);
+    if (rc) raise_status(rc, "options");
+    pybind11::dict d;
+    d["views"] = g.views;
+    d["dets"] = g.dets;
+    d["width"] = g.width;
+    d["height"] = g.height;
+    d["scan_type"] = g.scan_type;
+    d["dImg"] = g.dImg;
+    d["dDet"] = g.dDet;
+    d["Ang0"] = g.Ang0;
+    d["dAng"] = g.dAng;
+    d["s2r"] = g.s2r;
+    d["d2r"] = g.d2r;
+    d["binshift"] = g.binshift;
+    return d;
+}
+
+int64_t workspace_bytes(torch::Tensor options, int64_t op, int64_t batch, bool f64, int64_t device) {
+    Cached c = get_plan(options, (int)device);
+    return (int64_t)ctlib_workspace_bytes(c.plan.get(), (int)op, batch, f64 ? CTLIB_F64 : CTLIB_F32);
+}
+
+void clear_plan_cache() {
+    std::lock_guard<std::mutex> lock(g_mu);
+    g_cache.clear();
+}
+
+}  // namespace
+
+PYBIND11_MODULE(TORCH_EXTENSION_NAME, m) {
+    m.def("projection", &projection, "CT projection (distance driven, HIP)");
+    m.def("projection_t", &projection_t, "Transpose of CT projection (HIP)");
+    m.def("backprojection_t", &backprojection_t, "Transpose of backprojection (HIP)");
+    m.def("backprojection", &backprojection, "CT backprojection (HIP)");
+    m.def("fbp", &fbp, "CT filtered backprojection with RL filter (HIP)");
+    m.def("backprojection_t_true", &backprojection_t_true, "Exact adjoint of backprojection (HIP)");
+    m.def("fbp_t", &fbp_t, "Exact adjoint of fbp (HIP)");
+    m.def("residual_step", &residual_step, "(A x, A^T (A x - m)) in one call (HIP)");
+    m.def("parse_options", &parse_options, "Parse an options vector");
+    m.def("workspace_bytes", &workspace_bytes, "Scratch bytes of one op");
+    m.def("clear_plan_cache", &clear_plan_cache, "Drop all cached geometry plans");
+    m.attr("OP_PRJ") = (int)CTLIB_OP_PRJ;
+    m.attr("OP_PRJ_T") = (int)CTLIB_OP_PRJ_T;
+    m.attr("OP_BPRJ") = (int)CTLIB_OP_BPRJ;
+    m.attr("OP_BPRJ_T") = (int)CTLIB_OP_BPRJ_T;
+    m.attr("OP_BPRJ_T_TRUE") = (int)CTLIB_OP_BPRJ_T_TRUE;
+    m.attr("OP_FBP") = (int)CTLIB_OP_FBP;
+    m.attr("OP_FBP_T") = (int)CTLIB_OP_FBP_T;
+}

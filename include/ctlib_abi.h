@@ -1,0 +1,113 @@
+/*
+ * C ABI of the MI355X (gfx950) distance-driven CT projector library
+ * (libctlib_amd.so).  Plain C types only: no torch, no C++ exceptions.
+ *
+ * It replaces the reference's C++/CUDA dispatch layer:
+ *   ctlib.cpp:15-99           projection / projection_t / backprojection /
+ *                             backprojection_t / fbp  (scan_type switch)
+ *   fan_ed_cuda.h:6-11, fan_ea_cuda.h:6-10, para_cuda.h:5-9
+ *                             the per-geometry *_cuda host wrappers
+ * The reference re-parses the device-resident options tensor on every call
+ * (six .item() syncs, fan_ed_kernel.cu:1177-1181); here the options are parsed
+ * once into an immutable plan that holds the per-view geometry tables on the
+ * device, and ctlib_run only launches kernels on the caller's stream.
+ *
+ * Data layout (same as the reference's contiguous tensors, channel dim = 1):
+ *   image    [batch][height][width]
+ *   sinogram [batch][views][dets]     (bin 0 at the detector's negative end)
+ */
+#ifndef CTLIB_ABI_H
+#define CTLIB_ABI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum ctlib_status {
+    CTLIB_OK = 0,
+    CTLIB_ERR_INVALID_ARG = 1,     /* bad pointer, size, or option value           */
+    CTLIB_ERR_BAD_SCAN_TYPE = 2,   /* scan_type not in {0,1,2}; the reference exits */
+    CTLIB_ERR_SHAPE = 3,           /* tensor shape does not match the options       */
+    CTLIB_ERR_HIP = 4,             /* a HIP runtime call failed                      */
+    CTLIB_ERR_UNSUPPORTED_DTYPE = 5,
+    CTLIB_ERR_WORKSPACE = 6,       /* workspace missing or too small                 */
+    CTLIB_ERR_GEOMETRY = 7         /* geometry the DD method cannot represent        */
+} ctlib_status;
+
+typedef enum ctlib_dtype { CTLIB_F32 = 0, CTLIB_F64 = 1 } ctlib_dtype;
+
+typedef enum ctlib_op {
+    CTLIB_OP_PRJ = 0,          /* projection        image -> sinogram  (ctlib.cpp:56-69) */
+    CTLIB_OP_PRJ_T = 1,        /* projection_t      sinogram -> image  (ctlib.cpp:71-84) */
+    CTLIB_OP_BPRJ = 2,         /* backprojection    sinogram -> image  (ctlib.cpp:30-43) */
+    CTLIB_OP_BPRJ_T = 3,       /* backprojection_t  image -> sinogram  (ctlib.cpp:15-28),
+                                  including fan-EA's extra dImg/c_k factor
+                                  (fan_ea_kernel.cu:590-607)                            */
+    CTLIB_OP_BPRJ_T_TRUE = 4,  /* exact adjoint of CTLIB_OP_BPRJ (differs from
+                                  CTLIB_OP_BPRJ_T only for scan_type 1); no reference
+                                  counterpart, used by autograd                           */
+    CTLIB_OP_FBP = 5,          /* fbp               sinogram -> image  (ctlib.cpp:86-99) */
+    CTLIB_OP_FBP_T = 6         /* exact adjoint of CTLIB_OP_FBP; no reference counterpart */
+} ctlib_op;
+
+/* Geometry as read from the options vector (README.md:56-76). */
+typedef struct ctlib_geom {
+    int32_t views, dets, width, height, scan_type;
+    double dImg, dDet, Ang0, dAng, s2r, d2r, binshift;
+} ctlib_geom;
+
+typedef struct ctlib_plan ctlib_plan;
+
+/* Parse an options vector of n entries (values widened to double from the
+ * tensor dtype).  Index layout of fan_ed_kernel.cu:1165-1175 (fan, >= 11
+ * entries) and para_kernel.cu:687-695 (parallel, >= 9 entries, binshift =
+ * entry 8); scan_type is always the LAST entry (ctlib.cpp:18), so the README's
+ * 9-entry parallel vector reads binshift = 2, as the reference does. */
+int ctlib_parse_options(const double* opts, int n, ctlib_geom* out);
+
+/* Build the plan (FP64 host tables, uploaded to `device`).  The plan is
+ * immutable and may be shared by threads and streams on that device. */
+int ctlib_plan_create(const ctlib_geom* geom, int device, ctlib_plan** out);
+void ctlib_plan_destroy(ctlib_plan* plan);
+
+/* Scratch bytes ctlib_run needs for (op, batch, dtype).  0 is possible. */
+size_t ctlib_workspace_bytes(const ctlib_plan* plan, int op, int64_t batch, int dtype);
+
+/* Run one operator on `stream` (a hipStream_t; NULL = legacy default stream).
+ * Asynchronous: no host synchronisation, no allocation; `out` is fully
+ * written (no caller memset).  `in`/`out` are device pointers laid out as
+ * above; `ws` is device scratch of at least ctlib_workspace_bytes(). */
+int ctlib_run(const ctlib_plan* plan, int op, int dtype, const void* in, void* out, int64_t batch,
+              void* ws, size_t ws_bytes, void* stream);
+
+/* One least-squares data-term step, the unit of work of an unrolled
+ * reconstruction:  y = A x  and  grad = A^T (A x - m).  `y` may be NULL.
+ * No reference counterpart (the reference has no fused path). */
+int ctlib_run_residual(const ctlib_plan* plan, int dtype, const void* x, const void* m, void* y,
+                       void* grad, int64_t batch, void* ws, size_t ws_bytes, void* stream);
+size_t ctlib_residual_workspace_bytes(const ctlib_plan* plan, int64_t batch, int dtype);
+
+/* Geometry queries for host code that mirrors the plan (tests, autograd). */
+int ctlib_plan_geom(const ctlib_plan* plan, ctlib_geom* out);
+int ctlib_plan_view_info(const ctlib_plan* plan, int view, int* xmode, int* rev);
+
+/* Host copy of one plan table, built without touching a GPU (for host-side
+ * tooling and tests).  table: 0 view params (64-byte records), 1 bin edges
+ * [V][D+1] f64, 2 projection factors [V][D] f64, 3 backprojection_t factors
+ * [V][D] f64, 4 line coefficients [V][max(H,W)] (64-byte records), 5 X-axis
+ * view list int32, 6 Y-axis view list int32, 7 RL taps [2D-1] f64, 8 fbp
+ * pre-weight [D] f64.  With dst == NULL only *bytes is set. */
+int ctlib_plan_export(const ctlib_geom* geom, int table, void* dst, size_t* bytes);
+
+/* Last error text set on the calling thread (never NULL). */
+const char* ctlib_last_error(void);
+const char* ctlib_status_string(int status);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CTLIB_ABI_H */

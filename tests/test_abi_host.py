@@ -1,0 +1,129 @@
+"""Host-side checks of libctlib_amd.so: it loads, exports every symbol the
+public header declares, parses options like the reference, and its FP64
+geometry plan - run through a numpy model of the kernels' arithmetic - agrees
+with the FP64 oracle.  No GPU is touched."""
+import math
+import os
+import re
+
+import numpy as np
+import pytest
+
+from ctlib_amd import abi, phantom
+from oracle import oracle as O
+
+import kernel_model as M
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_functions():
+    text = open(os.path.join(ROOT, "include", "ctlib_abi.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(ctlib_[a-z_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    declared = header_functions()
+    assert len(declared) >= 10
+    lib = abi.load()
+    missing = [f for f in declared if not hasattr(lib, f)]
+    assert not missing, missing
+    assert sorted(abi.EXPORTS) == declared
+
+
+def test_status_strings():
+    lib = abi.load()
+    for code in range(8):
+        assert lib.ctlib_status_string(code)
+
+
+@pytest.mark.parametrize("opts,code", [
+    ([736, 736, 512, 512, 0.0066, 0.0072, 0, 0.01, 5.95, 4.906, 0, 3], 2),     # bad scan type
+    ([736, 736, 512, 512, 0.0066, 0.0072, 0, 0.01, 5.95, 4.906, 0, 0.5], 2),
+    ([736, 736, 512, 512, 0.0066, 0.0072, 0, 0.01, 0, 0], 1),                 # fan, too short
+    ([736, 736.5, 512, 512, 0.0066, 0.0072, 0, 0.01, 5.95, 4.906, 0, 0], 1),   # non-integral size
+    ([736, 736, 512, 512, -0.0066, 0.0072, 0, 0.01, 5.95, 4.906, 0, 0], 1),    # negative pixel
+    ([736, 736, 512, 512, 0.0066, 0.0072, 0, 0.01, 2], 2 - 2),                  # 9-entry para: valid
+])
+def test_parse_errors(opts, code):
+    if code == 0:
+        g = abi.parse_options(opts)
+        assert g.scan_type == 2 and g.binshift == 2.0   # README 9-entry vector: binshift = scan_type
+        return
+    with pytest.raises(abi.CtlibError) as e:
+        abi.parse_options(opts)
+    assert e.value.status == code
+
+
+def test_plan_rejects_source_inside_image():
+    opts = phantom.fan_options(16, 16, 64, 64, 0.1, 0.1, 2.0, 1.0, 0)  # source at r=2 inside a 6.4 wide image
+    with pytest.raises(abi.CtlibError) as e:
+        abi.plan_export(opts, "edges")
+    assert e.value.status == 7
+
+
+@pytest.mark.parametrize("st", [0, 1, 2])
+def test_plan_axis_decisions_match_oracle(st):
+    """Axis and reversal flags per view, including views on pi/4 ties."""
+    if st == 2:
+        o = phantom.para_options(8, 19, 10, 10, 0.1, 0.1, Ang0=0.0, dAng=math.pi / 4)
+    else:
+        o = phantom.fan_options(8, 21, 10, 10, 0.1, 0.03 if st == 1 else 0.13, 3.0, 2.0, st, Ang0=0.0,
+                                dAng=math.pi / 4)
+    vp = abi.plan_export(o, "views")
+    edges = abi.plan_export(o, "edges").reshape(8, -1)
+    for v in range(8):
+        xm, rev, e, _ = O.view(o.astype(np.float64), v)
+        assert bool(vp[v]["xmode"]) == xm and bool(vp[v]["rev"]) == rev
+        np.testing.assert_allclose(edges[v], e, rtol=1e-14, atol=1e-15)
+
+
+def test_tie_views_of_measured_configs():
+    """Views within 1e-7 of the pi/4 axis threshold (config 0 views 45, 135; config 4 views 180, 540)
+    take the axis the FP64 oracle takes."""
+    for cfg, views in ((0, (45, 135)), (4, (180, 540))):
+        o = phantom.CONFIGS[cfg]["options"]
+        vp = abi.plan_export(o, "views")
+        for v in views:
+            xm, rev, _, _ = O.view(o.astype(np.float64), v)
+            assert bool(vp[v]["xmode"]) == xm and bool(vp[v]["rev"]) == rev
+
+
+def small(st, **kw):
+    V, D, W, H = kw.get("V", 24), kw.get("D", 17), kw.get("W", 12), kw.get("H", 10)
+    if st == 2:
+        return phantom.para_options(V, D, W, H, 0.1, 0.13, Ang0=kw.get("Ang0", 0.1), dAng=2 * math.pi / V,
+                                    binshift=kw.get("bs", 0.02))
+    return phantom.fan_options(V, D, W, H, 0.1, 0.03 if st == 1 else 0.13, 3.0, 2.0, st, Ang0=kw.get("Ang0", 0.1),
+                               binshift=kw.get("bs", 0.02))
+
+
+@pytest.mark.parametrize("st", [0, 1, 2])
+@pytest.mark.parametrize("op", ["projection", "projection_t", "backprojection", "backprojection_t",
+                                "backprojection_t_true", "fbp"])
+def test_kernel_arithmetic_model_matches_oracle(st, op):
+    """The kernels' cumulative-integral arithmetic on the exported plan equals the oracle's merge."""
+    o = small(st, V=30, D=19, W=14, H=11, Ang0=0.25, bs=-0.03)
+    P = M.Plan(o)
+    g = P.g
+    rng = np.random.default_rng(12)
+    if op in ("projection", "backprojection_t", "backprojection_t_true"):
+        x = rng.random((2, g["height"], g["width"]))
+    else:
+        x = rng.random((2, g["views"], g["dets"]))
+    got = M.apply(P, op, x)
+    ref = O.apply(op, o.astype(np.float64), x)
+    err = np.linalg.norm(got - ref) / np.linalg.norm(ref)
+    tol = 1e-13 if op in ("projection", "projection_t") or st == 2 else 1e-7
+    assert err < tol, (op, st, err)
+
+
+def test_kernel_model_parallel_rows_at_full_config0():
+    """Config 0 (parallel 256^2, 180 views, 367 bins): model vs oracle on a few views' worth of data."""
+    o = phantom.CONFIGS[0]["options"]
+    P = M.Plan(o)
+    x = phantom.phantom_batch(1, 256, 256, config=0)[:, 0].astype(np.float64)
+    got = M.apply(P, "projection", x)
+    ref = O.apply("projection", o.astype(np.float64), x)
+    assert np.linalg.norm(got - ref) / np.linalg.norm(ref) < 1e-13

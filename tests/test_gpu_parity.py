@@ -1,0 +1,210 @@
+"""GPU parity of every operator against the FP64 oracle (oracle/ct_oracle.cpp).
+
+Bars (fp32 data): relative L2 <= 1e-5 for every operator; adjoint dot-product
+identities <= 1e-6 relative.  fp64 data: relative L2 <= 1e-7 (the pixel
+weights of backprojection / backprojection_t use a fp32 reciprocal).
+"""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+import ctlib_amd
+from ctlib_amd import phantom
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+F32_TOL = 1e-5
+F64_TOL = 1e-7
+ADJ_TOL = 1e-6
+
+IMG_IN = ("projection", "backprojection_t", "backprojection_t_true", "fbp_t")
+OPS = ("projection", "projection_t", "backprojection", "backprojection_t", "backprojection_t_true", "fbp", "fbp_t")
+
+
+def small_options(st, V=24, D=17, W=12, H=10, Ang0=0.1, bs=0.02):
+    if st == 2:
+        return phantom.para_options(V, D, W, H, 0.1, 0.13, Ang0=Ang0, dAng=2 * math.pi / V, binshift=bs)
+    return phantom.fan_options(V, D, W, H, 0.1, 0.03 if st == 1 else 0.13, 3.0, 2.0, st, Ang0=Ang0, binshift=bs)
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm((a - b).ravel()) / max(np.linalg.norm(b.ravel()), 1e-300))
+
+
+def gpu_op(op, x, options, dev):
+    t = torch.from_numpy(np.ascontiguousarray(x)).to(dev)
+    o = torch.from_numpy(np.asarray(options))
+    return getattr(ctlib_amd, op)(t, o).cpu().numpy()
+
+
+def oracle_op(op, x, options):
+    if op == "fbp_t":
+        # adjoint of fbp = preweight * ramp(dAng/2 * B_true^T x)
+        z = O.apply("backprojection_t_true", options, x[:, 0]) * (float(options[7]) / 2)
+        pw, delta = O.fbp_preweight(options)
+        h = O.rl_taps(z.shape[-1], delta)
+        D = z.shape[-1]
+        T = h[(np.arange(D)[:, None] - np.arange(D)[None, :]) + D - 1]
+        return ((z @ T) * pw)[:, None]
+    return O.apply(op, options, x[:, 0])[:, None]
+
+
+def inputs(op, options, B, rng, dtype=np.float32):
+    g = O.parse(options)
+    shape = (B, 1, g["height"], g["width"]) if op in IMG_IN else (B, 1, g["views"], g["dets"])
+    return rng.random(shape).astype(dtype)
+
+
+@pytest.mark.parametrize("st", [0, 1, 2])
+@pytest.mark.parametrize("op", OPS)
+def test_small_geometry_f32(dev, st, op):
+    rng = np.random.default_rng(100 + st)
+    o = small_options(st)
+    x = inputs(op, o, 3, rng)
+    print(f"[case] small st={st} op={op} f32", flush=True)
+    got = gpu_op(op, x, o, dev)
+    ref = oracle_op(op, x.astype(np.float64), o.astype(np.float64))
+    assert got.dtype == np.float32 and got.shape == ref.shape
+    assert rel_l2(got, ref) <= F32_TOL, (op, st, rel_l2(got, ref))
+
+
+@pytest.mark.parametrize("st", [0, 1, 2])
+@pytest.mark.parametrize("op", OPS)
+def test_small_geometry_f64(dev, st, op):
+    rng = np.random.default_rng(200 + st)
+    o = small_options(st, V=40, D=23, W=16, H=20, Ang0=0.3, bs=-0.05)
+    x = inputs(op, o, 2, rng, np.float64)
+    print(f"[case] small st={st} op={op} f64", flush=True)
+    got = gpu_op(op, x, o.astype(np.float64), dev)
+    ref = oracle_op(op, x, o.astype(np.float64))
+    assert got.dtype == np.float64
+    assert rel_l2(got, ref) <= F64_TOL, (op, st, rel_l2(got, ref))
+
+
+@pytest.mark.parametrize("op", OPS)
+def test_config0_parallel(dev, op):
+    """Parallel beam, 256^2, 180 views over pi, 367 bins (ellipse phantom input)."""
+    o = phantom.CONFIGS[0]["options"]
+    rng = np.random.default_rng(7)
+    if op in IMG_IN:
+        x = phantom.phantom_batch(1, 256, 256, config=0)
+    else:
+        clean = O.apply("projection", o, phantom.phantom_batch(1, 256, 256, config=0)[:, 0].astype(np.float64))
+        x = (clean[:, None] + 0.01 * rng.random(clean[:, None].shape)).astype(np.float32)
+    print(f"[case] config0 op={op}", flush=True)
+    got = gpu_op(op, x, o, dev)
+    ref = oracle_op(op, x.astype(np.float64), o.astype(np.float64))
+    err = rel_l2(got, ref)
+    print(f"  rel_l2={err:.3e} max_abs={np.abs(got - ref).max():.3e}")
+    assert err <= F32_TOL
+
+
+@pytest.mark.parametrize("cfg", [1, 2])
+@pytest.mark.parametrize("op", ["projection", "projection_t", "backprojection", "backprojection_t", "fbp"])
+def test_full_fan_geometry(dev, cfg, op):
+    """Fan-ED 512^2 736x736 and fan-EA 512^2 1024x888 geometries (one batch slice)."""
+    o = phantom.CONFIGS[cfg]["options"]
+    g = O.parse(o)
+    x_img = phantom.phantom_batch(1, g["height"], g["width"], config=cfg, scale=0.02)
+    if op in IMG_IN:
+        x = x_img
+    else:
+        x = O.apply("projection", o, x_img[:, 0].astype(np.float64))[:, None].astype(np.float32)
+    print(f"[case] config{cfg} op={op}", flush=True)
+    got = gpu_op(op, x, o, dev)
+    ref = oracle_op(op, x.astype(np.float64), o.astype(np.float64))
+    err = rel_l2(got, ref)
+    print(f"  rel_l2={err:.3e} max_abs={np.abs(got - ref).max():.3e}")
+    assert err <= F32_TOL
+
+
+@pytest.mark.parametrize("st", [0, 1, 2])
+def test_adjoint_dot(dev, st):
+    o = small_options(st, V=64, D=41, W=32, H=24)
+    g = O.parse(o)
+    rng = np.random.default_rng(3)
+    x = torch.from_numpy(rng.random((2, 1, g["height"], g["width"])).astype(np.float32)).to(dev)
+    y = torch.from_numpy(rng.random((2, 1, g["views"], g["dets"])).astype(np.float32)).to(dev)
+    ot = torch.from_numpy(o)
+    print(f"[case] adjoint st={st}", flush=True)
+    pairs = [("projection", "projection_t"), ("backprojection_t_true", "backprojection"), ("fbp_t", "fbp")]
+    for fwd, adj in pairs:
+        lhs = float((getattr(ctlib_amd, fwd)(x, ot).double() * y.double()).sum())
+        rhs = float((x.double() * getattr(ctlib_amd, adj)(y, ot).double()).sum())
+        assert abs(lhs - rhs) <= ADJ_TOL * max(abs(lhs), abs(rhs)), (fwd, adj, lhs, rhs)
+
+
+def test_deterministic(dev):
+    o = phantom.CONFIGS[1]["options"]
+    x = torch.from_numpy(phantom.phantom_batch(2, 512, 512, config=1, scale=0.02)).to(dev)
+    ot = torch.from_numpy(o)
+    print("[case] determinism", flush=True)
+    y1 = ctlib_amd.projection(x, ot)
+    y2 = ctlib_amd.projection(x, ot)
+    assert torch.equal(y1, y2)
+    a1 = ctlib_amd.projection_t(y1, ot)
+    a2 = ctlib_amd.projection_t(y1, ot)
+    assert torch.equal(a1, a2)
+
+
+def test_device_options_and_channels(dev):
+    o = small_options(0)
+    g = O.parse(o)
+    rng = np.random.default_rng(5)
+    x = torch.from_numpy(rng.random((2, 3, g["height"], g["width"])).astype(np.float32)).to(dev)
+    print("[case] device options / channel 0", flush=True)
+    host = ctlib_amd.projection(x[:, :1].contiguous(), torch.from_numpy(o))
+    devo = ctlib_amd.projection(x, torch.from_numpy(o).to(dev))   # options on the device, 3 channels
+    assert devo.shape == host.shape
+    assert torch.equal(host, devo)
+
+
+def test_para_nine_entry_options_binshift_quirk(dev):
+    """The README's 9-entry parallel vector reads binshift = options[8] = scan_type = 2."""
+    o10 = phantom.para_options(30, 21, 16, 16, 0.1, 0.1, binshift=2.0)
+    o9 = np.concatenate([o10[:8], [2.0]]).astype(np.float32)
+    rng = np.random.default_rng(9)
+    x = rng.random((1, 1, 16, 16)).astype(np.float32)
+    print("[case] 9-entry para options", flush=True)
+    a = gpu_op("projection", x, o9, dev)
+    b = gpu_op("projection", x, o10, dev)
+    np.testing.assert_array_equal(a, b)
+    ref = oracle_op("projection", x.astype(np.float64), o9.astype(np.float64))
+    assert rel_l2(a, ref) <= F32_TOL
+
+
+def test_errors(dev):
+    o = torch.from_numpy(small_options(0))
+    x = torch.zeros(1, 1, 10, 12, device=dev)
+    print("[case] errors", flush=True)
+    with pytest.raises(RuntimeError):
+        ctlib_amd.projection(torch.zeros(1, 1, 10, 12), o)           # CPU tensor
+    with pytest.raises(RuntimeError):
+        ctlib_amd.projection(torch.zeros(1, 1, 12, 10, device=dev).transpose(2, 3), o)  # non-contiguous
+    with pytest.raises(ValueError):
+        ctlib_amd.projection(torch.zeros(1, 1, 11, 12, device=dev), o)  # shape mismatch
+    bad = o.clone()
+    bad[-1] = 7
+    with pytest.raises(ValueError):
+        ctlib_amd.projection(x, bad)
+    with pytest.raises(TypeError):
+        ctlib_amd.projection(x.half(), o)
+    assert ctlib_amd.projection(torch.zeros(0, 1, 10, 12, device=dev), o).shape == (0, 1, 24, 17)
+
+
+def test_residual_step(dev):
+    o = small_options(1, V=48, D=31, W=20, H=20)
+    g = O.parse(o)
+    rng = np.random.default_rng(11)
+    x = torch.from_numpy(rng.random((2, 1, 20, 20)).astype(np.float32)).to(dev)
+    m = torch.from_numpy(rng.random((2, 1, g["views"], g["dets"])).astype(np.float32)).to(dev)
+    ot = torch.from_numpy(o)
+    print("[case] residual_step", flush=True)
+    y, grad = ctlib_amd.residual_step(x, m, ot)
+    y_ref = ctlib_amd.projection(x, ot)
+    g_ref = ctlib_amd.projection_t(y_ref - m, ot)
+    assert torch.equal(y, y_ref)
+    assert rel_l2(grad.cpu().numpy(), g_ref.cpu().numpy()) <= 1e-6
