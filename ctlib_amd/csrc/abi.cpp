@@ -16,6 +16,7 @@
 #include <mutex>
 #include <new>
 #include <string>
+#include <vector>
 
 #include "../../include/ctlib_abi.h"
 #include "kernels.h"
@@ -121,13 +122,68 @@ WsLayout layout(const ctlib_plan* p, int op, int64_t B, int dtype) {
     return L;
 }
 
-FwdShape default_fwd_shape(const DevPlan& P, int mode, bool weighted) {
+FwdShape default_fwd_shape(const DevPlan& P, bool weighted) {
     // Largest view group whose LDS footprint keeps >= 2 blocks per CU.
     for (int G : {8, 4, 2}) {
         if (weighted && G != 2) continue;
-        if (fwd_lds_bytes(P, mode, weighted, G) <= 72 * 1024) return FwdShape{G};
+        if (fwd_lds_bytes(P, weighted, G) <= 72 * 1024) return FwdShape{G};
     }
     return FwdShape{2};
+}
+
+// ---- kernel timing (HIP event pairs around the main launches, on their stream)
+enum { K_FWD = 0, K_ADJ = 1, K_RAMP = 2, K_AUX = 3 };
+
+struct TimedLaunch {
+    hipEvent_t a, b;
+    int cls;
+    double bytes, flops;
+};
+
+std::mutex g_tmu;
+bool g_timing = false;
+std::vector<TimedLaunch> g_rec;
+std::vector<hipEvent_t> g_pool;
+
+hipEvent_t pool_get() {
+    if (!g_pool.empty()) {
+        hipEvent_t e = g_pool.back();
+        g_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+class KernelTimer {
+  public:
+    KernelTimer(int cls, double bytes, double flops, hipStream_t s) : s_(s) {
+        std::lock_guard<std::mutex> lock(g_tmu);
+        if (!g_timing) return;
+        t_ = TimedLaunch{pool_get(), pool_get(), cls, bytes, flops};
+        on_ = t_.a && t_.b && hipEventRecord(t_.a, s_) == hipSuccess;
+    }
+    void done() {
+        if (!on_) return;
+        on_ = false;
+        if (hipEventRecord(t_.b, s_) != hipSuccess) return;
+        std::lock_guard<std::mutex> lock(g_tmu);
+        g_rec.push_back(t_);
+    }
+    ~KernelTimer() { done(); }
+
+  private:
+    hipStream_t s_;
+    TimedLaunch t_{};
+    bool on_ = false;
+};
+
+// Algorithmic bytes of one DD operator application (sweep model): every view streams
+// the whole image once and its own sinogram row, es * B * V * (H*W + D).
+double sweep_bytes(const DevPlan& P, int B, int dt) {
+    const double es = dt == DT_F64 ? 8.0 : 4.0;
+    return es * B * (double)P.V * ((double)P.H * P.W + P.D);
 }
 
 int check_run_args(const ctlib_plan* p, int op, int dtype, const void* in, void* out, int64_t B, void* ws,
@@ -148,15 +204,22 @@ hipError_t run_to_sino(const ctlib_plan* p, int dtype, const void* in, void* out
                        const double* post, const WsLayout& L, char* ws, hipStream_t s) {
     const DevPlan& P = p->dev;
     hipError_t e = hipSuccess;
-    if (P.nx) e = launch_fwd(P, dtype, in, out, MODE_X, B, weighted, post, default_fwd_shape(P, MODE_X, weighted), s);
-    if (e != hipSuccess) return e;
+    void* yl = nullptr;
     if (P.ny) {
-        void* yl = ws + L.ylines;
+        yl = ws + L.ylines;
+        KernelTimer t(K_AUX, 0, 0, s);
         e = launch_to_ylines(in, yl, dtype, B, P.H, P.W, s);
         if (e != hipSuccess) return e;
-        e = launch_fwd(P, dtype, yl, out, MODE_Y, B, weighted, post, default_fwd_shape(P, MODE_Y, weighted), s);
     }
-    return e;
+    KernelTimer t(K_FWD, sweep_bytes(P, B, dtype), 0, s);
+    return launch_fwd(P, dtype, in, yl, out, B, weighted, post, default_fwd_shape(P, weighted), s);
+}
+
+// Bin tables for the image-output ops.
+hipError_t run_tables(const ctlib_plan* p, int in_dtype, const void* y, const void* m, const double* pre,
+                      double2* tabs, int B, hipStream_t s) {
+    KernelTimer t(K_AUX, 0, 0, s);
+    return launch_bin_tables(p->dev, in_dtype, y, m, pre, 1.0, tabs, B, s);
 }
 
 // Image-output ops from prepared bin tables: projection_t and backprojection.
@@ -164,13 +227,22 @@ hipError_t run_to_image(const ctlib_plan* p, int dtype, const double2* tabs, voi
                         double wscale, const WsLayout& L, char* ws, hipStream_t s) {
     const DevPlan& P = p->dev;
     void* yt = ws + L.yt;
-    hipError_t e = hipSuccess;
-    if (P.nx) e = launch_adj(P, dtype, tabs, out, nullptr, MODE_X, B, weighted, wscale, AdjShape{0}, s);
-    if (e != hipSuccess) return e;
-    if (P.ny) e = launch_adj(P, dtype, tabs, nullptr, yt, MODE_Y, B, weighted, wscale, AdjShape{0}, s);
-    if (e != hipSuccess) return e;
-    if (P.ny) e = launch_combine(out, yt, dtype, B, P.H, P.W, P.nx > 0, true, s);
-    return e;
+    hipError_t e;
+    {
+        KernelTimer t(K_ADJ, sweep_bytes(P, B, dtype), 0, s);
+        e = launch_adj(P, dtype, tabs, out, yt, B, weighted, wscale, AdjShape{0}, s);
+    }
+    if (e != hipSuccess || !P.ny) return e;
+    KernelTimer t(K_AUX, 0, 0, s);
+    return launch_combine(out, yt, dtype, B, P.H, P.W, P.nx > 0, true, s);
+}
+
+hipError_t run_ramp(const ctlib_plan* p, int in_dt, const void* in, int out_dt, void* out, int B, const double* pre,
+                    const double* post, double scale, hipStream_t s) {
+    const DevPlan& P = p->dev;
+    const long rows = (long)B * P.V;
+    KernelTimer t(K_RAMP, 0, 2.0 * rows * (double)P.D * P.D, s);
+    return launch_ramp(P, in_dt, in, out_dt, out, rows, pre, post, scale, s);
 }
 
 }  // namespace
@@ -403,26 +475,26 @@ int ctlib_run(const ctlib_plan* p, int op, int dtype, const void* in, void* out,
         e = run_to_sino(p, dt, in, out, B, true, nullptr, L, w, s);
         break;
     case CTLIB_OP_PRJ_T:
-        e = launch_bin_tables(P, dt, in, nullptr, P.post_prj, 1.0, tabs, B, s);
+        e = run_tables(p, dt, in, nullptr, P.post_prj, tabs, B, s);
         if (e == hipSuccess) e = run_to_image(p, dt, tabs, out, B, false, 1.0, L, w, s);
         break;
     case CTLIB_OP_BPRJ:
-        e = launch_bin_tables(P, dt, in, nullptr, nullptr, 1.0, tabs, B, s);
+        e = run_tables(p, dt, in, nullptr, nullptr, tabs, B, s);
         if (e == hipSuccess) e = run_to_image(p, dt, tabs, out, B, true, 1.0, L, w, s);
         break;
     case CTLIB_OP_FBP: {
         // pre-weight + RL ramp (Toeplitz GEMM) -> weighted backprojection * dAng/2 (fan_ed_kernel.cu:1349-1380)
         double* rows = reinterpret_cast<double*>(w + L.rows);
-        e = launch_ramp(P, dt, in, DT_F64, rows, (long)B * P.V, P.preweight, nullptr, 1.0, s);
-        if (e == hipSuccess) e = launch_bin_tables(P, DT_F64, rows, nullptr, nullptr, 1.0, tabs, B, s);
+        e = run_ramp(p, dt, in, DT_F64, rows, B, P.preweight, nullptr, 1.0, s);
+        if (e == hipSuccess) e = run_tables(p, DT_F64, rows, nullptr, nullptr, tabs, B, s);
         if (e == hipSuccess) e = run_to_image(p, dt, tabs, out, B, true, P.dAng / 2, L, w, s);
         break;
     }
     case CTLIB_OP_FBP_T: {
-        // adjoint of fbp: exact backprojection adjoint * dAng/2, then ramp, then pre-weight
+        // adjoint of fbp: exact backprojection adjoint, then ramp * dAng/2, then pre-weight
         void* rows = w + L.rows;
         e = run_to_sino(p, dt, in, rows, B, true, nullptr, L, w, s);
-        if (e == hipSuccess) e = launch_ramp(P, dt, rows, dt, out, (long)B * P.V, nullptr, P.preweight, P.dAng / 2, s);
+        if (e == hipSuccess) e = run_ramp(p, dt, rows, dt, out, B, nullptr, P.preweight, P.dAng / 2, s);
         break;
     }
     default:
@@ -430,6 +502,36 @@ int ctlib_run(const ctlib_plan* p, int op, int dtype, const void* in, void* out,
     }
     if (e != hipSuccess) return hip_fail(e, "kernel launch");
     return CTLIB_OK;
+}
+
+int ctlib_timing_enable(int on) {
+    std::lock_guard<std::mutex> lock(g_tmu);
+    g_timing = on != 0;
+    return CTLIB_OK;
+}
+
+int ctlib_timing_collect(ctlib_kernel_stats* out) {
+    if (!out) return fail(CTLIB_ERR_INVALID_ARG, "NULL argument");
+    std::memset(out, 0, sizeof(*out));
+    std::lock_guard<std::mutex> lock(g_tmu);
+    int rc = CTLIB_OK;
+    for (const TimedLaunch& t : g_rec) {
+        float ms = 0.f;
+        hipError_t e = hipEventSynchronize(t.b);
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, t.a, t.b);
+        if (e != hipSuccess) {
+            rc = hip_fail(e, "timing events");
+        } else {
+            out->launches[t.cls] += 1;
+            out->ms[t.cls] += ms;
+            out->algo_bytes[t.cls] += t.bytes;
+            out->flops[t.cls] += t.flops;
+        }
+        g_pool.push_back(t.a);
+        g_pool.push_back(t.b);
+    }
+    g_rec.clear();
+    return rc;
 }
 
 size_t ctlib_residual_workspace_bytes(const ctlib_plan* p, int64_t batch, int dtype) {
@@ -461,7 +563,7 @@ int ctlib_run_residual(const ctlib_plan* p, int dtype, const void* x, const void
     void* ybuf = y ? y : static_cast<void*>(w + base);
     hipError_t e = run_to_sino(p, dt, x, ybuf, B, false, P.post_prj, la, w, s);
     double2* tabs = reinterpret_cast<double2*>(w + lb.tables);
-    if (e == hipSuccess) e = launch_bin_tables(P, dt, ybuf, m, P.post_prj, 1.0, tabs, B, s);
+    if (e == hipSuccess) e = run_tables(p, dt, ybuf, m, P.post_prj, tabs, B, s);
     if (e == hipSuccess) e = run_to_image(p, dt, tabs, grad, B, false, 1.0, lb, w, s);
     if (e != hipSuccess) return hip_fail(e, "kernel launch");
     return CTLIB_OK;

@@ -132,15 +132,24 @@ __device__ __forceinline__ void line_table(const double (&val)[MAXM], int m, int
     if (tid == NT - 1) tb[n + 1] = make_double2(ws[0] + ws[1] + ws[2] + ws[3], 0.0);
 }
 
+// One launch covers both sweep axes: work items [0, B*ngx) are (b, group of G
+// X-axis views) and read image rows from `srcx`; the rest are Y-axis groups
+// reading the bottom-to-top column copy `srcy`.
 template <typename T, int G, int NE, bool WEIGHTED>
-__global__ __launch_bounds__(256) void k_fwd(DevPlan P, const T* __restrict__ src, T* __restrict__ y, int mode,
-                                             int ngroups, int nwork, const double* __restrict__ post) {
+__global__ __launch_bounds__(256) void k_fwd(DevPlan P, const T* __restrict__ srcx, const T* __restrict__ srcy,
+                                             T* __restrict__ y, int B, int ngx, int ngy,
+                                             const double* __restrict__ post) {
     extern __shared__ __align__(16) unsigned char smem[];
     constexpr int NTAB = WEIGHTED ? G : 1;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int w = xcd_remap(blockIdx.x, gridDim.x, nwork);
+    const int nwx = B * ngx;
+    int w = xcd_remap(blockIdx.x, gridDim.x, nwx + B * ngy);
     if (w < 0) return;
+    const int mode = w < nwx ? MODE_X : MODE_Y;
+    if (mode == MODE_Y) w -= nwx;
+    const int ngroups = mode == MODE_Y ? ngy : ngx;
     const int b = w / ngroups, grp = w - b * ngroups;
+    const T* src = mode == MODE_Y ? srcy : srcx;
     const int* vlist = mode == MODE_Y ? P.yviews : P.xviews;
     const int nv = mode == MODE_Y ? P.ny : P.nx;
     const int nlines = mode == MODE_Y ? P.W : P.H;
@@ -310,15 +319,22 @@ __device__ __forceinline__ double eval_G(const double2* __restrict__ t, double a
     return __builtin_fma(pi, c.y, c.x);
 }
 
+// One launch covers both sweep axes: work items [0, B*ntx) are (b, tile of
+// 4*LPW image rows) over the X-axis views, the rest (b, tile of columns) over
+// the Y-axis views, written bottom-to-top to outYT.
 template <typename T, int LPW, int NPE, bool WEIGHTED, int GEO>
 __global__ __launch_bounds__(256) void k_adj(DevPlan P, const double2* __restrict__ tabs, T* __restrict__ outX,
-                                             T* __restrict__ outYT, int mode, int ntiles, int nwork, double wscale) {
+                                             T* __restrict__ outYT, int B, int ntx, int nty, double wscale) {
     extern __shared__ __align__(16) unsigned char smem[];
     double2* stab = reinterpret_cast<double2*>(smem);  // [D+2]
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-    const int w = xcd_remap(blockIdx.x, gridDim.x, nwork);
+    const int nwx = B * ntx;
+    int w = xcd_remap(blockIdx.x, gridDim.x, nwx + B * nty);
     if (w < 0) return;
+    const int mode = w < nwx ? MODE_X : MODE_Y;
+    if (mode == MODE_Y) w -= nwx;
+    const int ntiles = mode == MODE_Y ? nty : ntx;
     const int b = w / ntiles, tile = w - b * ntiles;
     const int* vlist = mode == MODE_Y ? P.yviews : P.xviews;
     const int nv = mode == MODE_Y ? P.ny : P.nx;
@@ -466,10 +482,11 @@ hipError_t launch_to_ylines(const void* x, void* xT, int dtype, int B, int H, in
     return hipGetLastError();
 }
 
-size_t fwd_lds_bytes(const DevPlan& P, int mode, bool weighted, int G) {
-    const int n = mode == MODE_Y ? P.H : P.W;
+size_t fwd_lds_bytes(const DevPlan& P, bool weighted, int G) {
+    const int n = P.nx ? P.W : P.H;
+    const int nmax = P.ny && P.H > n ? P.H : n;
     const int ntab = weighted ? G : 1;
-    const size_t tabs = (size_t)2 * ntab * (n + 2) * sizeof(double2) + (size_t)2 * ntab * 4 * sizeof(double);
+    const size_t tabs = (size_t)2 * ntab * (nmax + 2) * sizeof(double2) + (size_t)2 * ntab * 4 * sizeof(double);
     const size_t ex = (size_t)G * (P.D + 1) * sizeof(double);
     return tabs > ex ? tabs : ex;
 }
@@ -477,59 +494,57 @@ size_t fwd_lds_bytes(const DevPlan& P, int mode, bool weighted, int G) {
 namespace {
 
 template <typename T, int G, int NE, bool WTD>
-hipError_t fwd_launch_t(const DevPlan& P, const void* src, void* y, int mode, int B, const double* post,
+hipError_t fwd_launch_t(const DevPlan& P, const void* srcx, const void* srcy, void* y, int B, const double* post,
                         hipStream_t s) {
-    const int nv = mode == MODE_Y ? P.ny : P.nx;
-    const int ngroups = (nv + G - 1) / G;
-    const int nwork = ngroups * B;
-    const size_t lds = fwd_lds_bytes(P, mode, WTD, G);
+    const int ngx = (P.nx + G - 1) / G, ngy = (P.ny + G - 1) / G;
+    const int nwork = (ngx + ngy) * B;
+    const size_t lds = fwd_lds_bytes(P, WTD, G);
     auto kern = k_fwd<T, G, NE, WTD>;
     if (lds > 64 * 1024) {
         hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL(kern, dim3(grid_for(nwork)), dim3(NT), lds, s, P, (const T*)src, (T*)y, mode, ngroups, nwork,
-                       post);
+    hipLaunchKernelGGL(kern, dim3(grid_for(nwork)), dim3(NT), lds, s, P, (const T*)srcx, (const T*)srcy, (T*)y, B,
+                       ngx, ngy, post);
     return hipGetLastError();
 }
 
 template <typename T, int G, bool WTD>
-hipError_t fwd_ne(const DevPlan& P, const void* src, void* y, int mode, int B, const double* post, hipStream_t s) {
+hipError_t fwd_ne(const DevPlan& P, const void* sx, const void* sy, void* y, int B, const double* post,
+                  hipStream_t s) {
     const int ne = (P.D + 1 + NT - 1) / NT;
     switch (ne) {
-    case 1: return fwd_launch_t<T, G, 1, WTD>(P, src, y, mode, B, post, s);
-    case 2: return fwd_launch_t<T, G, 2, WTD>(P, src, y, mode, B, post, s);
-    case 3: return fwd_launch_t<T, G, 3, WTD>(P, src, y, mode, B, post, s);
-    case 4: return fwd_launch_t<T, G, 4, WTD>(P, src, y, mode, B, post, s);
+    case 1: return fwd_launch_t<T, G, 1, WTD>(P, sx, sy, y, B, post, s);
+    case 2: return fwd_launch_t<T, G, 2, WTD>(P, sx, sy, y, B, post, s);
+    case 3: return fwd_launch_t<T, G, 3, WTD>(P, sx, sy, y, B, post, s);
+    case 4: return fwd_launch_t<T, G, 4, WTD>(P, sx, sy, y, B, post, s);
     case 5:
-    case 6: return fwd_launch_t<T, G, 6, WTD>(P, src, y, mode, B, post, s);
+    case 6: return fwd_launch_t<T, G, 6, WTD>(P, sx, sy, y, B, post, s);
     case 7:
-    case 8: return fwd_launch_t<T, G, 8, WTD>(P, src, y, mode, B, post, s);
+    case 8: return fwd_launch_t<T, G, 8, WTD>(P, sx, sy, y, B, post, s);
     default: return hipErrorInvalidValue;
     }
 }
 
 template <typename T>
-hipError_t fwd_dispatch(const DevPlan& P, const void* src, void* y, int mode, int B, bool weighted, const double* post,
-                        int G, hipStream_t s) {
-    if (weighted) return fwd_ne<T, 2, true>(P, src, y, mode, B, post, s);
+hipError_t fwd_dispatch(const DevPlan& P, const void* sx, const void* sy, void* y, int B, bool weighted,
+                        const double* post, int G, hipStream_t s) {
+    if (weighted) return fwd_ne<T, 2, true>(P, sx, sy, y, B, post, s);
     switch (G) {
-    case 2: return fwd_ne<T, 2, false>(P, src, y, mode, B, post, s);
-    case 8: return fwd_ne<T, 8, false>(P, src, y, mode, B, post, s);
-    default: return fwd_ne<T, 4, false>(P, src, y, mode, B, post, s);
+    case 2: return fwd_ne<T, 2, false>(P, sx, sy, y, B, post, s);
+    case 8: return fwd_ne<T, 8, false>(P, sx, sy, y, B, post, s);
+    default: return fwd_ne<T, 4, false>(P, sx, sy, y, B, post, s);
     }
 }
 
 }  // namespace
 
-hipError_t launch_fwd(const DevPlan& P, int dtype, const void* src, void* y, int mode, int B, bool weighted,
+hipError_t launch_fwd(const DevPlan& P, int dtype, const void* srcx, const void* srcy, void* y, int B, bool weighted,
                       const double* post, FwdShape shape, hipStream_t s) {
-    const int nv = mode == MODE_Y ? P.ny : P.nx;
-    if (B <= 0 || nv == 0) return hipSuccess;
-    const int n = mode == MODE_Y ? P.H : P.W;
-    if (n > NT * MAXM) return hipErrorInvalidValue;
-    if (dtype == DT_F32) return fwd_dispatch<float>(P, src, y, mode, B, weighted, post, shape.G, s);
-    return fwd_dispatch<double>(P, src, y, mode, B, weighted, post, shape.G, s);
+    if (B <= 0 || P.nx + P.ny == 0) return hipSuccess;
+    if ((P.nx && P.W > NT * MAXM) || (P.ny && P.H > NT * MAXM)) return hipErrorInvalidValue;
+    if (dtype == DT_F32) return fwd_dispatch<float>(P, srcx, srcy, y, B, weighted, post, shape.G, s);
+    return fwd_dispatch<double>(P, srcx, srcy, y, B, weighted, post, shape.G, s);
 }
 
 hipError_t launch_bin_tables(const DevPlan& P, int in_dtype, const void* y, const void* m, const double* pre,
@@ -549,56 +564,57 @@ hipError_t launch_bin_tables(const DevPlan& P, int in_dtype, const void* y, cons
 namespace {
 
 template <typename T, int LPW, int NPE, bool WTD, int GEO>
-hipError_t adj_launch_t(const DevPlan& P, const double2* tabs, void* outX, void* outYT, int mode, int B, double wscale,
+hipError_t adj_launch_t(const DevPlan& P, const double2* tabs, void* outX, void* outYT, int B, double wscale,
                         hipStream_t s) {
-    const int nlines = mode == MODE_Y ? P.W : P.H;
-    const int ntiles = (nlines + 4 * LPW - 1) / (4 * LPW);
-    const int nwork = ntiles * B;
+    const int L = 4 * LPW;
+    const int ntx = P.nx ? (P.H + L - 1) / L : 0;
+    const int nty = P.ny ? (P.W + L - 1) / L : 0;
+    const int nwork = (ntx + nty) * B;
     const size_t lds = (size_t)(P.D + 2) * sizeof(double2);
     auto kern = k_adj<T, LPW, NPE, WTD, GEO>;
     if (lds > 64 * 1024) {
         hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL(kern, dim3(grid_for(nwork)), dim3(NT), lds, s, P, tabs, (T*)outX, (T*)outYT, mode, ntiles, nwork,
+    hipLaunchKernelGGL(kern, dim3(grid_for(nwork)), dim3(NT), lds, s, P, tabs, (T*)outX, (T*)outYT, B, ntx, nty,
                        wscale);
     return hipGetLastError();
 }
 
 template <typename T, bool WTD, int GEO>
-hipError_t adj_npe(const DevPlan& P, const double2* tabs, void* outX, void* outYT, int mode, int B, double wscale,
+hipError_t adj_npe(const DevPlan& P, const double2* tabs, void* outX, void* outYT, int B, double wscale,
                    hipStream_t s) {
-    const int n = mode == MODE_Y ? P.H : P.W;
+    int n = P.nx ? P.W : 0;
+    if (P.ny && P.H > n) n = P.H;
     const int npe = (n + 1 + 63) / 64;
-    if (npe <= 5) return adj_launch_t<T, 8, 5, WTD, GEO>(P, tabs, outX, outYT, mode, B, wscale, s);
-    if (npe <= 9) return adj_launch_t<T, 4, 9, WTD, GEO>(P, tabs, outX, outYT, mode, B, wscale, s);
-    if (npe <= 17) return adj_launch_t<T, 2, 17, WTD, GEO>(P, tabs, outX, outYT, mode, B, wscale, s);
-    if (npe <= 33) return adj_launch_t<T, 1, 33, WTD, GEO>(P, tabs, outX, outYT, mode, B, wscale, s);
+    if (npe <= 5) return adj_launch_t<T, 8, 5, WTD, GEO>(P, tabs, outX, outYT, B, wscale, s);
+    if (npe <= 9) return adj_launch_t<T, 4, 9, WTD, GEO>(P, tabs, outX, outYT, B, wscale, s);
+    if (npe <= 17) return adj_launch_t<T, 2, 17, WTD, GEO>(P, tabs, outX, outYT, B, wscale, s);
+    if (npe <= 33) return adj_launch_t<T, 1, 33, WTD, GEO>(P, tabs, outX, outYT, B, wscale, s);
     return hipErrorInvalidValue;
 }
 
 template <typename T, bool WTD>
-hipError_t adj_geo(const DevPlan& P, const double2* tabs, void* outX, void* outYT, int mode, int B, double wscale,
+hipError_t adj_geo(const DevPlan& P, const double2* tabs, void* outX, void* outYT, int B, double wscale,
                    hipStream_t s) {
     switch (P.scan_type) {
-    case GEO_ED: return adj_npe<T, WTD, GEO_ED>(P, tabs, outX, outYT, mode, B, wscale, s);
-    case GEO_EA: return adj_npe<T, WTD, GEO_EA>(P, tabs, outX, outYT, mode, B, wscale, s);
-    default: return adj_npe<T, WTD, GEO_PARA>(P, tabs, outX, outYT, mode, B, wscale, s);
+    case GEO_ED: return adj_npe<T, WTD, GEO_ED>(P, tabs, outX, outYT, B, wscale, s);
+    case GEO_EA: return adj_npe<T, WTD, GEO_EA>(P, tabs, outX, outYT, B, wscale, s);
+    default: return adj_npe<T, WTD, GEO_PARA>(P, tabs, outX, outYT, B, wscale, s);
     }
 }
 
 }  // namespace
 
-hipError_t launch_adj(const DevPlan& P, int dtype, const double2* tabs, void* outX, void* outYT, int mode, int B,
-                      bool weighted, double wscale, AdjShape, hipStream_t s) {
-    const int nv = mode == MODE_Y ? P.ny : P.nx;
-    if (B <= 0 || nv == 0) return hipSuccess;
+hipError_t launch_adj(const DevPlan& P, int dtype, const double2* tabs, void* outX, void* outYT, int B, bool weighted,
+                      double wscale, AdjShape, hipStream_t s) {
+    if (B <= 0 || P.nx + P.ny == 0) return hipSuccess;
     if (dtype == DT_F32) {
-        return weighted ? adj_geo<float, true>(P, tabs, outX, outYT, mode, B, wscale, s)
-                        : adj_geo<float, false>(P, tabs, outX, outYT, mode, B, wscale, s);
+        return weighted ? adj_geo<float, true>(P, tabs, outX, outYT, B, wscale, s)
+                        : adj_geo<float, false>(P, tabs, outX, outYT, B, wscale, s);
     }
-    return weighted ? adj_geo<double, true>(P, tabs, outX, outYT, mode, B, wscale, s)
-                    : adj_geo<double, false>(P, tabs, outX, outYT, mode, B, wscale, s);
+    return weighted ? adj_geo<double, true>(P, tabs, outX, outYT, B, wscale, s)
+                    : adj_geo<double, false>(P, tabs, outX, outYT, B, wscale, s);
 }
 
 hipError_t launch_combine(void* out, const void* yt, int dtype, int B, int H, int W, bool hasX, bool hasY,

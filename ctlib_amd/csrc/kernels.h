@@ -34,19 +34,19 @@ struct AdjShape { int lpw; };     // lines per wave of the bin->pixel kernel (0 
 // x [B][H][W] -> xT [B][W][H] with xT[b][c][i] = x[b][H-1-i][c]  (Y-mode lines, bottom to top)
 hipError_t launch_to_ylines(const void* x, void* xT, int dtype, int B, int H, int W, hipStream_t s);
 
-// Line -> bin operator (projection; weighted = backprojection_t) for the views of one sweep axis.
-// `src` holds the lines of that axis ([B][H][W] for X, the transposed copy for Y).
-hipError_t launch_fwd(const DevPlan& P, int dtype, const void* src, void* y, int mode, int B, bool weighted,
-                      const double* post, FwdShape shape, hipStream_t s);
-size_t fwd_lds_bytes(const DevPlan& P, int mode, bool weighted, int G);
+// Line -> bin operator (projection; weighted = backprojection_t), both sweep axes in one launch.
+// `srcx` holds image rows ([B][H][W]), `srcy` the bottom-to-top column copy ([B][W][H]).
+hipError_t launch_fwd(const DevPlan& P, int dtype, const void* srcx, const void* srcy, void* y, int B,
+                      bool weighted, const double* post, FwdShape shape, hipStream_t s);
+size_t fwd_lds_bytes(const DevPlan& P, bool weighted, int G);
 
 // Per (b, view) cumulative-integral tables {C_j, g_j}, j = 0..D+1, of g_k = (y - m)[o(k)] * pre[k] * scale.
 hipError_t launch_bin_tables(const DevPlan& P, int in_dtype, const void* y, const void* m, const double* pre,
                              double scale, double2* tabs, int B, hipStream_t s);
 
-// Bin -> pixel operator (projection_t; weighted = backprojection, pixel weight * wscale) for one axis.
-// X lines are written to outX [B][H][W]; Y lines to outYT [B][W][H] (bottom-to-top per column).
-hipError_t launch_adj(const DevPlan& P, int dtype, const double2* tabs, void* outX, void* outYT, int mode, int B,
+// Bin -> pixel operator (projection_t; weighted = backprojection, pixel weight * wscale), both axes in
+// one launch.  X lines are written to outX [B][H][W]; Y lines to outYT [B][W][H] (bottom-to-top per column).
+hipError_t launch_adj(const DevPlan& P, int dtype, const double2* tabs, void* outX, void* outYT, int B,
                       bool weighted, double wscale, AdjShape shape, hipStream_t s);
 
 // out[b][r][c] = (hasX ? out[b][r][c] : 0) + (hasY ? yt[b][c][H-1-r] : 0)

@@ -19,9 +19,9 @@ namespace {
 typedef double dbl4 __attribute__((ext_vector_type(4)));
 
 // Block: 4 waves, 64 rows x 64 columns; each wave owns 16 rows x 64 columns
-// (four 16x16 accumulators).  Operand layout of v_mfma_f64_16x16x4f64:
-// A[i][k] and B[k][j] are held by lane i + 16k (resp. j + 16k); accumulator
-// element r of lane l is D[4*(l/16) + r][l%16].
+// (four 16x16 accumulators).  Operand layout of v_mfma_f64_16x16x4f64 (checked
+// on gfx950): A[i][k] and B[k][j] are held by lane i + 16k (resp. j + 16k);
+// accumulator element r of lane l is D[l/16 + 4r][l%16].
 template <typename TIn, typename TOut>
 __global__ __launch_bounds__(256) void k_ramp(const TIn* __restrict__ in, TOut* __restrict__ out, long rows, int D,
                                              const double* __restrict__ taps, const double* __restrict__ pre,
@@ -60,7 +60,7 @@ __global__ __launch_bounds__(256) void k_ramp(const TIn* __restrict__ in, TOut* 
         const double f = post ? post[col] * scale : scale;
 #pragma unroll
         for (int r = 0; r < 4; r++) {
-            const long row = r0 + 4 * kk + r;
+            const long row = r0 + kk + 4 * r;
             if (row < rows) out[row * D + col] = (TOut)(acc[t][r] * f);
         }
     }

@@ -211,6 +211,27 @@ int64_t workspace_bytes(torch::Tensor options, int64_t op, int64_t batch, bool f
     return (int64_t)ctlib_workspace_bytes(c.plan.get(), (int)op, batch, f64 ? CTLIB_F64 : CTLIB_F32);
 }
 
+void timing_enable(bool on) { ctlib_timing_enable(on ? 1 : 0); }
+
+// Per kernel class: launches, total milliseconds, algorithmic bytes and flops of the
+// launches recorded since the last collect (classes: fwd, adj, ramp, aux).
+pybind11::dict timing_collect() {
+    ctlib_kernel_stats st{};
+    int rc = ctlib_timing_collect(&st);
+    if (rc) raise_status(rc, "timing_collect");
+    static const char* names[4] = {"fwd", "adj", "ramp", "aux"};
+    pybind11::dict d;
+    for (int i = 0; i < 4; i++) {
+        pybind11::dict e;
+        e["launches"] = st.launches[i];
+        e["ms"] = st.ms[i];
+        e["algo_bytes"] = st.algo_bytes[i];
+        e["flops"] = st.flops[i];
+        d[names[i]] = e;
+    }
+    return d;
+}
+
 void clear_plan_cache() {
     std::lock_guard<std::mutex> lock(g_mu);
     g_cache.clear();
@@ -230,6 +251,8 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m) {
     m.def("parse_options", &parse_options, "Parse an options vector");
     m.def("workspace_bytes", &workspace_bytes, "Scratch bytes of one op");
     m.def("clear_plan_cache", &clear_plan_cache, "Drop all cached geometry plans");
+    m.def("timing_enable", &timing_enable, "Record HIP events around the library's kernel launches");
+    m.def("timing_collect", &timing_collect, "Per-class kernel times since the last collect");
     m.attr("OP_PRJ") = (int)CTLIB_OP_PRJ;
     m.attr("OP_PRJ_T") = (int)CTLIB_OP_PRJ_T;
     m.attr("OP_BPRJ") = (int)CTLIB_OP_BPRJ;

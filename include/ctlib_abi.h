@@ -102,6 +102,22 @@ int ctlib_plan_view_info(const ctlib_plan* plan, int view, int* xmode, int* rev)
  * pre-weight [D] f64.  With dst == NULL only *bytes is set. */
 int ctlib_plan_export(const ctlib_geom* geom, int table, void* dst, size_t* bytes);
 
+/* Kernel timing.  While enabled, ctlib_run / ctlib_run_residual record a HIP
+ * event pair on the launch stream around each launch, grouped in classes:
+ * 0 line->bin DD kernel, 1 bin->pixel DD kernel, 2 ramp GEMM, 3 auxiliary
+ * (line transposes, bin tables, axis combine).  ctlib_timing_collect waits for
+ * the recorded events, sums them per class and clears the record.
+ * algo_bytes: sweep-model bytes of the DD kernels, es*B*V*(H*W + D) per launch;
+ * flops: 2*B*V*D*D per ramp launch.  No reference counterpart. */
+typedef struct ctlib_kernel_stats {
+    int64_t launches[4];
+    double ms[4];
+    double algo_bytes[4];
+    double flops[4];
+} ctlib_kernel_stats;
+int ctlib_timing_enable(int on);
+int ctlib_timing_collect(ctlib_kernel_stats* out);
+
 /* Last error text set on the calling thread (never NULL). */
 const char* ctlib_last_error(void);
 const char* ctlib_status_string(int status);
