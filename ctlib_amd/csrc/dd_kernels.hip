@@ -104,15 +104,17 @@ __global__ __launch_bounds__(256) void k_to_ylines(const T* __restrict__ x, T* _
 // ---------------------------------------------------------------------------
 // Line -> bin (projection / backprojection_t)
 // ---------------------------------------------------------------------------
-constexpr int MAXM = 8;  // pixels per thread in the line scan: n <= 2048
+constexpr int LC_WORDS = 8;  // doubles per LineCoef: a, s, A, B, w0, w1, w2, inv_s
+enum { LC_A0 = 0, LC_S = 1, LC_A = 2, LC_B = 3, LC_W0 = 4, LC_W1 = 5, LC_W2 = 6, LC_INVS = 7 };
 
 // Builds one line table {C_j, slope_j}, j = 0..n+1, from `val` (this thread's
 // pixels p0 .. p0+m-1).  `ws` holds 4 wave sums; one barrier inside.
-__device__ __forceinline__ void line_table(const double (&val)[MAXM], int m, int p0, int n, double2* __restrict__ tb,
+template <int M>
+__device__ __forceinline__ void line_table(const double (&val)[M], int m, int p0, int n, double2* __restrict__ tb,
                                            double* __restrict__ ws, int tid, int lane, int wave) {
     double run = 0;
 #pragma unroll
-    for (int i = 0; i < MAXM; i++)
+    for (int i = 0; i < M; i++)
         if (i < m) run += val[i];
     const double inc = wave_scan_incl(run, lane);
     if (lane == 63) ws[wave] = inc;
@@ -121,7 +123,7 @@ __device__ __forceinline__ void line_table(const double (&val)[MAXM], int m, int
     for (int q = 0; q < wave; q++) base += ws[q];
     double pref = base + inc - run;  // sum of all pixels before p0
 #pragma unroll
-    for (int i = 0; i < MAXM; i++) {
+    for (int i = 0; i < M; i++) {
         if (i < m) {
             const int p = p0 + i;
             if (p < n) tb[p + 1] = make_double2(pref - (double)(p + 1) * val[i], val[i]);
@@ -135,7 +137,13 @@ __device__ __forceinline__ void line_table(const double (&val)[MAXM], int m, int
 // One launch covers both sweep axes: work items [0, B*ngx) are (b, group of G
 // X-axis views) and read image rows from `srcx`; the rest are Y-axis groups
 // reading the bottom-to-top column copy `srcy`.
-template <typename T, int G, int NE, bool WEIGHTED>
+//
+// Per image line: the line is loaded one iteration ahead into registers, its
+// cumulative table is built in LDS (double-buffered), and the G views'
+// LineCoefs for the line (prefetched one line ahead by G*8 threads with vector
+// loads, so no scalar-cache round trip sits between LDS reads) are staged in
+// LDS next to it.  Each thread then evaluates its NE bin edges for every view.
+template <typename T, int G, int NE, int M, bool WEIGHTED>
 __global__ __launch_bounds__(256) void k_fwd(DevPlan P, const T* __restrict__ srcx, const T* __restrict__ srcy,
                                              T* __restrict__ y, int B, int ngx, int ngy,
                                              const double* __restrict__ post) {
@@ -174,54 +182,71 @@ __global__ __launch_bounds__(256) void k_fwd(DevPlan P, const T* __restrict__ sr
     }
 
     const int tabn = n + 2;
-    double2* tab = reinterpret_cast<double2*>(smem);                // [2][NTAB][tabn]
-    double* wsum = reinterpret_cast<double*>(tab + 2 * NTAB * tabn);  // [2][NTAB][4]
+    double2* tab = reinterpret_cast<double2*>(smem);               // [2][NTAB][tabn]
+    double* cf = reinterpret_cast<double*>(tab + 2 * NTAB * tabn);  // [2][G][LC_WORDS]
+    double* wsum = cf + 2 * G * LC_WORDS;                           // [2][NTAB][4]
     const T* base = src + (size_t)b * nlines * n;
     const int m = (n + NT - 1) / NT;
     const int p0 = tid * m;
     const float hi = (float)(n + 1);
 
-    T xv[MAXM];
+    // coefficient word `cw` of view `cg` for the next line, held by thread tid < G*8
+    const bool cthread = tid < G * LC_WORDS;
+    const int cg = tid / LC_WORDS, cw = tid % LC_WORDS;
+    const double* crow = nullptr;
+    if (cthread) {
+        const int idx = grp * G + cg;
+        crow = reinterpret_cast<const double*>(P.lines + (size_t)vlist[idx < nv ? idx : nv - 1] * P.max_lines) + cw;
+    }
+    double cnext = cthread ? crow[0] : 0.0;
+
+    T xv[M];
 #pragma unroll
-    for (int i = 0; i < MAXM; i++) xv[i] = (i < m && p0 + i < n) ? base[p0 + i] : T(0);
+    for (int i = 0; i < M; i++) xv[i] = (i < m && p0 + i < n) ? base[p0 + i] : T(0);
 
     for (int l = 0; l < nlines; l++) {
         double2* tb = tab + (l & 1) * NTAB * tabn;
         double* ws = wsum + (l & 1) * NTAB * 4;
+        double* cfl = cf + (l & 1) * G * LC_WORDS;
+        if (cthread) cfl[tid] = cnext;
+        if (WEIGHTED) __syncthreads();  // the weights below read cfl
 #pragma unroll
         for (int t = 0; t < NTAB; t++) {
-            double val[MAXM];
+            double val[M];
             if (WEIGHTED) {
-                const LineCoef& lc = P.lines[(size_t)vg[t] * P.max_lines + l];
+                const double w0 = cfl[t * LC_WORDS + LC_W0], w1 = cfl[t * LC_WORDS + LC_W1];
+                const double w2 = cfl[t * LC_WORDS + LC_W2], is = cfl[t * LC_WORDS + LC_INVS];
 #pragma unroll
-                for (int i = 0; i < MAXM; i++) {
+                for (int i = 0; i < M; i++) {
                     val[i] = 0.0;
                     if (i < m && p0 + i < n) {
-                        const double u = __builtin_fma(lc.w1, (double)(p0 + i), lc.w0);
-                        const double d = __builtin_fma(u, u, lc.w2);
-                        val[i] = (double)xv[i] * (lc.inv_s * (double)__builtin_amdgcn_rcpf((float)d));
+                        const double u = __builtin_fma(w1, (double)(p0 + i), w0);
+                        const double d = __builtin_fma(u, u, w2);
+                        val[i] = (double)xv[i] * (is * (double)__builtin_amdgcn_rcpf((float)d));
                     }
                 }
             } else {
 #pragma unroll
-                for (int i = 0; i < MAXM; i++) val[i] = (i < m && p0 + i < n) ? (double)xv[i] : 0.0;
+                for (int i = 0; i < M; i++) val[i] = (i < m && p0 + i < n) ? (double)xv[i] : 0.0;
             }
-            line_table(val, m, p0, n, tb + t * tabn, ws + t * 4, tid, lane, wave);
+            line_table<M>(val, m, p0, n, tb + t * tabn, ws + t * 4, tid, lane, wave);
         }
         if (l + 1 < nlines) {
             const T* nb = base + (size_t)(l + 1) * n;
 #pragma unroll
-            for (int i = 0; i < MAXM; i++) xv[i] = (i < m && p0 + i < n) ? nb[p0 + i] : T(0);
+            for (int i = 0; i < M; i++) xv[i] = (i < m && p0 + i < n) ? nb[p0 + i] : T(0);
+            if (cthread) cnext = crow[(size_t)(l + 1) * LC_WORDS];
         }
         __syncthreads();
 #pragma unroll
         for (int g = 0; g < G; g++) {
-            const LineCoef& lc = P.lines[(size_t)vg[g] * P.max_lines + l];
-            const double A = lc.A, B = lc.B, s = lc.s;
+            const double s = cfl[g * LC_WORDS + LC_S];
+            const double A = cfl[g * LC_WORDS + LC_A];
+            const double Bc = cfl[g * LC_WORDS + LC_B];
             const double2* t = tb + (WEIGHTED ? g : 0) * tabn;
 #pragma unroll
             for (int i = 0; i < NE; i++) {
-                const double tau = __builtin_fma(A, e[g][i], B);
+                const double tau = __builtin_fma(A, e[g][i], Bc);
                 const int j = table_index((float)tau, hi);
                 const double2 c = t[j];
                 S[g][i] = __builtin_fma(s, __builtin_fma(tau, c.y, c.x), S[g][i]);
@@ -322,11 +347,16 @@ __device__ __forceinline__ double eval_G(const double2* __restrict__ t, double a
 // One launch covers both sweep axes: work items [0, B*ntx) are (b, tile of
 // 4*LPW image rows) over the X-axis views, the rest (b, tile of columns) over
 // the Y-axis views, written bottom-to-top to outYT.
+//
+// Per view: the next view's bin table is copied global -> LDS with
+// global_load_lds_dwordx4 (asynchronous, no register staging) and the next
+// view's LineCoefs of the tile are prefetched into one register per thread
+// while the current view is evaluated; one barrier per view.
 template <typename T, int LPW, int NPE, bool WEIGHTED, int GEO>
 __global__ __launch_bounds__(256) void k_adj(DevPlan P, const double2* __restrict__ tabs, T* __restrict__ outX,
                                              T* __restrict__ outYT, int B, int ntx, int nty, double wscale) {
     extern __shared__ __align__(16) unsigned char smem[];
-    double2* stab = reinterpret_cast<double2*>(smem);  // [D+2]
+    constexpr int L = 4 * LPW;
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int nwx = B * ntx;
@@ -343,6 +373,31 @@ __global__ __launch_bounds__(256) void k_adj(DevPlan P, const double2* __restric
     const int D = P.D;
     const int npe = (n + 1 + 63) / 64;  // edge groups (<= NPE)
     const float hi = (float)(D + 1);
+    const int tabn = D + 2;
+
+    double2* stab = reinterpret_cast<double2*>(smem);                   // [2][tabn]
+    double* scf = reinterpret_cast<double*>(stab + 2 * tabn);           // [2][L][LC_WORDS]
+
+    const double2* tb_b = tabs + (size_t)b * P.V * tabn;
+    // Asynchronous copy of view v's table into LDS buffer `buf`: wave w copies
+    // entries [64*(w + 4r), 64*(w + 4r) + 64); lanes past the end are masked off.
+    auto table_to_lds = [&](int v, int buf) {
+        const double2* src = tb_b + (size_t)v * tabn;
+        for (int j0 = wave * 64; j0 < tabn; j0 += NT) {
+            if (j0 + lane < tabn)
+                __builtin_amdgcn_global_load_lds(
+                    (const void*)(src + j0 + lane),
+                    (__attribute__((address_space(3))) void*)(stab + buf * tabn + j0), 16, 0, 0);
+        }
+    };
+    // LineCoef word `tid % 8` of tile line `tid / 8`, held by thread tid < L*8.
+    const bool cthread = tid < L * LC_WORDS;
+    const int cl = tile * L + tid / LC_WORDS;
+    const bool cvalid = cthread && cl < nlines;
+    auto coef_word = [&](int v) {
+        return cvalid ? reinterpret_cast<const double*>(P.lines + (size_t)v * P.max_lines + cl)[tid % LC_WORDS]
+                      : 0.0;
+    };
 
     double acc[LPW][NPE];
 #pragma unroll
@@ -350,28 +405,39 @@ __global__ __launch_bounds__(256) void k_adj(DevPlan P, const double2* __restric
 #pragma unroll
         for (int i = 0; i < NPE; i++) acc[q][i] = 0.0;
 
-    const double2* tb_b = tabs + (size_t)b * P.V * (D + 2);
+    table_to_lds(vlist[0], 0);
+    if (cthread) scf[tid] = coef_word(vlist[0]);
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    __syncthreads();
+
     for (int iv = 0; iv < nv; iv++) {
+        const int cur = iv & 1;
         const int v = vlist[iv];
-        __syncthreads();
-        const double2* src = tb_b + (size_t)v * (D + 2);
-        for (int j = tid; j < D + 2; j += NT) stab[j] = src[j];
-        __syncthreads();
+        double cnext = 0.0;
+        if (iv + 1 < nv) {
+            const int vn = vlist[iv + 1];
+            table_to_lds(vn, cur ^ 1);
+            cnext = coef_word(vn);
+        }
+        const double2* t = stab + cur * tabn;
+        const double* cf = scf + cur * L * LC_WORDS;
         const ViewParams& vp = P.vp[v];
         const float lk[4] = {vp.lk[0], vp.lk[1], vp.lk[2], vp.lk[3]};
         const float ea_sc = vp.ea_sc, ea_off = vp.ea_off;
 #pragma unroll
         for (int q = 0; q < LPW; q++) {
-            const int l = tile * (4 * LPW) + q * 4 + wave;
-            if (l >= nlines) continue;  // wave-uniform
-            const LineCoef& lc = P.lines[(size_t)v * P.max_lines + l];
-            const double a = lc.a, s = lc.s;
+            const int lq = q * 4 + wave;  // line within the tile
+            const int l = tile * L + lq;
+            if (l >= nlines) continue;    // wave-uniform
+            const double a = cf[lq * LC_WORDS + LC_A0], s = cf[lq * LC_WORDS + LC_S];
             if (!WEIGHTED) {
 #pragma unroll
                 for (int i = 0; i < NPE; i++)
-                    if (i < npe) acc[q][i] += eval_G<GEO>(stab, a, s, lane + 64 * i, lk, ea_sc, ea_off, hi);
+                    if (i < npe) acc[q][i] += eval_G<GEO>(t, a, s, lane + 64 * i, lk, ea_sc, ea_off, hi);
             } else {
-                const double w0 = lc.w0, w1 = lc.w1, w2 = lc.w2, ws = lc.inv_s * wscale;
+                const double w0 = cf[lq * LC_WORDS + LC_W0], w1 = cf[lq * LC_WORDS + LC_W1];
+                const double w2 = cf[lq * LC_WORDS + LC_W2];
+                const double ws = cf[lq * LC_WORDS + LC_INVS] * wscale;
                 auto wgt = [&](int p) {
                     const double u = __builtin_fma(w1, (double)p, w0);
                     const double d = __builtin_fma(u, u, w2);
@@ -381,7 +447,7 @@ __global__ __launch_bounds__(256) void k_adj(DevPlan P, const double2* __restric
 #pragma unroll
                 for (int i = 0; i < NPE; i++) {
                     if (i < npe) {
-                        const double gi = eval_G<GEO>(stab, a, s, lane + 64 * i, lk, ea_sc, ea_off, hi);
+                        const double gi = eval_G<GEO>(t, a, s, lane + 64 * i, lk, ea_sc, ea_off, hi);
                         if (i > 0) {
                             double right = shfl_down_d(gprev, 1);
                             const double r0 = readlane_d(gi, 0);
@@ -404,12 +470,15 @@ __global__ __launch_bounds__(256) void k_adj(DevPlan P, const double2* __restric
                 }
             }
         }
+        if (iv + 1 < nv && cthread) scf[(cur ^ 1) * L * LC_WORDS + tid] = cnext;
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        __syncthreads();
     }
 
     // Write pixel values.
 #pragma unroll
     for (int q = 0; q < LPW; q++) {
-        const int l = tile * (4 * LPW) + q * 4 + wave;
+        const int l = tile * L + q * 4 + wave;
         if (l >= nlines) continue;
         T* dst = mode == MODE_Y ? outYT + ((size_t)b * P.W + l) * P.H : outX + ((size_t)b * P.H + l) * P.W;
 #pragma unroll
@@ -486,20 +555,21 @@ size_t fwd_lds_bytes(const DevPlan& P, bool weighted, int G) {
     const int n = P.nx ? P.W : P.H;
     const int nmax = P.ny && P.H > n ? P.H : n;
     const int ntab = weighted ? G : 1;
-    const size_t tabs = (size_t)2 * ntab * (nmax + 2) * sizeof(double2) + (size_t)2 * ntab * 4 * sizeof(double);
+    const size_t tabs = (size_t)2 * ntab * (nmax + 2) * sizeof(double2) + (size_t)2 * G * LC_WORDS * sizeof(double) +
+                        (size_t)2 * ntab * 4 * sizeof(double);
     const size_t ex = (size_t)G * (P.D + 1) * sizeof(double);
     return tabs > ex ? tabs : ex;
 }
 
 namespace {
 
-template <typename T, int G, int NE, bool WTD>
+template <typename T, int G, int NE, int M, bool WTD>
 hipError_t fwd_launch_t(const DevPlan& P, const void* srcx, const void* srcy, void* y, int B, const double* post,
                         hipStream_t s) {
     const int ngx = (P.nx + G - 1) / G, ngy = (P.ny + G - 1) / G;
     const int nwork = (ngx + ngy) * B;
     const size_t lds = fwd_lds_bytes(P, WTD, G);
-    auto kern = k_fwd<T, G, NE, WTD>;
+    auto kern = k_fwd<T, G, NE, M, WTD>;
     if (lds > 64 * 1024) {
         hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
@@ -509,19 +579,31 @@ hipError_t fwd_launch_t(const DevPlan& P, const void* srcx, const void* srcy, vo
     return hipGetLastError();
 }
 
+template <typename T, int G, int NE, bool WTD>
+hipError_t fwd_m(const DevPlan& P, const void* sx, const void* sy, void* y, int B, const double* post, hipStream_t s) {
+    int n = P.nx ? P.W : 0;
+    if (P.ny && P.H > n) n = P.H;
+    const int m = (n + NT - 1) / NT;
+    if (m <= 1) return fwd_launch_t<T, G, NE, 1, WTD>(P, sx, sy, y, B, post, s);
+    if (m <= 2) return fwd_launch_t<T, G, NE, 2, WTD>(P, sx, sy, y, B, post, s);
+    if (m <= 4) return fwd_launch_t<T, G, NE, 4, WTD>(P, sx, sy, y, B, post, s);
+    if (m <= 8) return fwd_launch_t<T, G, NE, 8, WTD>(P, sx, sy, y, B, post, s);
+    return hipErrorInvalidValue;
+}
+
 template <typename T, int G, bool WTD>
 hipError_t fwd_ne(const DevPlan& P, const void* sx, const void* sy, void* y, int B, const double* post,
                   hipStream_t s) {
     const int ne = (P.D + 1 + NT - 1) / NT;
     switch (ne) {
-    case 1: return fwd_launch_t<T, G, 1, WTD>(P, sx, sy, y, B, post, s);
-    case 2: return fwd_launch_t<T, G, 2, WTD>(P, sx, sy, y, B, post, s);
-    case 3: return fwd_launch_t<T, G, 3, WTD>(P, sx, sy, y, B, post, s);
-    case 4: return fwd_launch_t<T, G, 4, WTD>(P, sx, sy, y, B, post, s);
+    case 1: return fwd_m<T, G, 1, WTD>(P, sx, sy, y, B, post, s);
+    case 2: return fwd_m<T, G, 2, WTD>(P, sx, sy, y, B, post, s);
+    case 3: return fwd_m<T, G, 3, WTD>(P, sx, sy, y, B, post, s);
+    case 4: return fwd_m<T, G, 4, WTD>(P, sx, sy, y, B, post, s);
     case 5:
-    case 6: return fwd_launch_t<T, G, 6, WTD>(P, sx, sy, y, B, post, s);
+    case 6: return fwd_m<T, G, 6, WTD>(P, sx, sy, y, B, post, s);
     case 7:
-    case 8: return fwd_launch_t<T, G, 8, WTD>(P, sx, sy, y, B, post, s);
+    case 8: return fwd_m<T, G, 8, WTD>(P, sx, sy, y, B, post, s);
     default: return hipErrorInvalidValue;
     }
 }
@@ -542,7 +624,7 @@ hipError_t fwd_dispatch(const DevPlan& P, const void* sx, const void* sy, void* 
 hipError_t launch_fwd(const DevPlan& P, int dtype, const void* srcx, const void* srcy, void* y, int B, bool weighted,
                       const double* post, FwdShape shape, hipStream_t s) {
     if (B <= 0 || P.nx + P.ny == 0) return hipSuccess;
-    if ((P.nx && P.W > NT * MAXM) || (P.ny && P.H > NT * MAXM)) return hipErrorInvalidValue;
+    if ((P.nx && P.W > NT * 8) || (P.ny && P.H > NT * 8)) return hipErrorInvalidValue;
     if (dtype == DT_F32) return fwd_dispatch<float>(P, srcx, srcy, y, B, weighted, post, shape.G, s);
     return fwd_dispatch<double>(P, srcx, srcy, y, B, weighted, post, shape.G, s);
 }
@@ -570,7 +652,7 @@ hipError_t adj_launch_t(const DevPlan& P, const double2* tabs, void* outX, void*
     const int ntx = P.nx ? (P.H + L - 1) / L : 0;
     const int nty = P.ny ? (P.W + L - 1) / L : 0;
     const int nwork = (ntx + nty) * B;
-    const size_t lds = (size_t)(P.D + 2) * sizeof(double2);
+    const size_t lds = (size_t)2 * (P.D + 2) * sizeof(double2) + (size_t)2 * L * LC_WORDS * sizeof(double);
     auto kern = k_adj<T, LPW, NPE, WTD, GEO>;
     if (lds > 64 * 1024) {
         hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
