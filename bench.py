@@ -69,6 +69,8 @@ def main():
     ap.add_argument("--batch", type=int, default=32)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--paths", default="plain,autograd,graph")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="run one 256 MiB device copy first (reference kernel for HBM counter scaling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; the projector shards over slices / views "
@@ -81,6 +83,12 @@ def main():
     from ctlib_amd import _C, autograd, phantom
 
     dev = torch.device("cuda:0")
+    if args.calib_copy:
+        src = torch.ones(64 << 20, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     B = args.batch
     opts = torch.from_numpy(workload_options())
     x_true = torch.from_numpy(phantom.phantom_batch(B, 512, 512, config=1, scale=0.02)).to(dev)

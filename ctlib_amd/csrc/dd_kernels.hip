@@ -53,7 +53,7 @@ __device__ __forceinline__ double ld(const T* p) { return (double)*p; }
 
 // Index of a cumulative table from a float coordinate, clamped to [0, hi].
 __device__ __forceinline__ int table_index(float q, float hi) {
-    return (int)__builtin_fminf(__builtin_fmaxf(q, 0.0f), hi);
+    return (int)__builtin_amdgcn_fmed3f(q, 0.0f, hi);
 }
 
 __device__ __forceinline__ double shfl_up_d(double v, int off) { return __shfl_up(v, off, 64); }
@@ -297,28 +297,29 @@ __global__ __launch_bounds__(256) void k_bin_tables(DevPlan P, const TIn* __rest
     const TIn* yr = y + (size_t)row * D;
     const TIn* mr = m ? m + (size_t)row * D : nullptr;
     const double* pr = pre ? pre + (size_t)v * D : nullptr;
-    auto gval = [&](int k) {
-        const int o = rev ? D - 1 - k : k;
-        double val = (double)yr[o];
-        if (mr) val -= (double)mr[o];
-        if (pr) val *= pr[k];
-        return val * scale;
-    };
-    const int per = (D + 63) / 64;
-    const int k0 = lane * per, k1 = min(k0 + per, D);
-    double run = 0;
-    for (int k = k0; k < k1; k++) run += (e[k + 1] - e[k]) * gval(k);
-    const double inc = wave_scan_incl(run, lane);
-    const double total = readlane_d(inc, 63);
-    double Q = inc - run;
     double2* tb = tabs + (size_t)row * (D + 2);
-    for (int k = k0; k < k1; k++) {
-        const double g = gval(k);
-        tb[k + 1] = make_double2(Q - e[k] * g, g);
-        Q += (e[k + 1] - e[k]) * g;
+    // 64 consecutive bins per step: coalesced loads and stores, one wave scan per step
+    double carry = 0.0;
+    for (int k0 = 0; k0 < D; k0 += 64) {
+        const int k = k0 + lane;
+        double g = 0.0, ek = 0.0, wg = 0.0;
+        if (k < D) {
+            const int o = rev ? D - 1 - k : k;
+            g = (double)yr[o];
+            if (mr) g -= (double)mr[o];
+            if (pr) g *= pr[k];
+            g *= scale;
+            ek = e[k];
+            wg = (e[k + 1] - ek) * g;
+        }
+        const double inc = wave_scan_incl(wg, lane);
+        if (k < D) tb[k + 1] = make_double2(carry + (inc - wg) - ek * g, g);
+        carry += readlane_d(inc, 63);
     }
-    if (lane == 0) tb[0] = make_double2(0.0, 0.0);
-    if (lane == 63) tb[D + 1] = make_double2(total, 0.0);
+    if (lane == 0) {
+        tb[0] = make_double2(0.0, 0.0);
+        tb[D + 1] = make_double2(carry, 0.0);
+    }
 }
 
 // ---------------------------------------------------------------------------
