@@ -63,6 +63,7 @@ def build(arch="gfx950", verbose=False, with_torch=True):
         "abi.o": [HIPCC, f"--offload-arch={arch}", *HIP_FLAGS, "-c", os.path.join(CSRC, "abi.cpp")],
         "dd_kernels.o": [HIPCC, f"--offload-arch={arch}", *HIP_FLAGS, "-c", os.path.join(CSRC, "dd_kernels.hip")],
         "ramp_mfma.o": [HIPCC, f"--offload-arch={arch}", *HIP_FLAGS, "-c", os.path.join(CSRC, "ramp_mfma.hip")],
+        "batch.o": [HIPCC, f"--offload-arch={arch}", *HIP_FLAGS, "-c", os.path.join(CSRC, "batch.hip")],
     }
     jobs = []
     for name, cmd in objs.items():

@@ -29,7 +29,12 @@ struct ctlib_plan {
     int device = 0;
     void* dmem = nullptr;
     DevPlan dev{};
+    int fwd_G = 0;  // autotuned views per line->bin block (0 = automatic)
 };
+
+namespace ctlib {
+DevPlan plan_dev(const ::ctlib_plan* p) { return p->dev; }
+}  // namespace ctlib
 
 namespace {
 
@@ -122,13 +127,11 @@ WsLayout layout(const ctlib_plan* p, int op, int64_t B, int dtype) {
     return L;
 }
 
-FwdShape default_fwd_shape(const DevPlan& P, bool weighted) {
-    // Largest view group whose LDS footprint keeps >= 2 blocks per CU.
-    for (int G : {8, 4, 2}) {
-        if (weighted && G != 2) continue;
-        if (fwd_lds_bytes(P, weighted, G) <= 72 * 1024) return FwdShape{G};
-    }
-    return FwdShape{2};
+FwdShape default_fwd_shape(const DevPlan& P, bool weighted, int tuned = 0) {
+    if (weighted) return FwdShape{4};
+    if (tuned == 4 || tuned == 8) return FwdShape{tuned};
+    // two views per wave when the edge slots fit (<= 12 per view), else one
+    return FwdShape{(P.D + 1 + 63) / 64 <= 12 ? 8 : 4};
 }
 
 // ---- kernel timing (HIP event pairs around the main launches, on their stream)
@@ -212,7 +215,7 @@ hipError_t run_to_sino(const ctlib_plan* p, int dtype, const void* in, void* out
         if (e != hipSuccess) return e;
     }
     KernelTimer t(K_FWD, sweep_bytes(P, B, dtype), 0, s);
-    return launch_fwd(P, dtype, in, yl, out, B, weighted, post, default_fwd_shape(P, weighted), s);
+    return launch_fwd(P, dtype, in, yl, out, B, weighted, post, default_fwd_shape(P, weighted, p->fwd_G), s);
 }
 
 // Bin tables for the image-output ops.
@@ -393,6 +396,14 @@ size_t ctlib_workspace_bytes(const ctlib_plan* p, int op, int64_t batch, int dty
     return layout(p, op, batch, dtype).total;
 }
 
+int ctlib_plan_set_tuning(ctlib_plan* p, int fwd_views_per_block) {
+    if (!p) return fail(CTLIB_ERR_INVALID_ARG, "NULL plan");
+    if (fwd_views_per_block != 0 && fwd_views_per_block != 4 && fwd_views_per_block != 8)
+        return fail(CTLIB_ERR_INVALID_ARG, "views per block must be 0 (automatic), 4 or 8");
+    p->fwd_G = fwd_views_per_block;
+    return CTLIB_OK;
+}
+
 int ctlib_plan_geom(const ctlib_plan* p, ctlib_geom* out) {
     if (!p || !out) return fail(CTLIB_ERR_INVALID_ARG, "NULL argument");
     const Geom& g = p->host.g;
@@ -538,8 +549,8 @@ size_t ctlib_residual_workspace_bytes(const ctlib_plan* p, int64_t batch, int dt
     if (!p || batch <= 0) return 0;
     const WsLayout a = layout(p, CTLIB_OP_PRJ, batch, dtype);
     const WsLayout b = layout(p, CTLIB_OP_PRJ_T, batch, dtype);
-    const size_t sino = align_up((size_t)batch * p->dev.V * p->dev.D * dsize(dtype));
-    return (a.total > b.total ? a.total : b.total) + sino;
+    const size_t tables = align_up((size_t)batch * p->dev.V * (p->dev.D + 2) * sizeof(double2));
+    return (a.total > b.total ? a.total : b.total) + tables;
 }
 
 int ctlib_run_residual(const ctlib_plan* p, int dtype, const void* x, const void* m, void* y, void* grad,
@@ -557,13 +568,22 @@ int ctlib_run_residual(const ctlib_plan* p, int dtype, const void* x, const void
     const int dt = dtype == CTLIB_F64 ? DT_F64 : DT_F32;
     hipStream_t s = static_cast<hipStream_t>(stream);
     char* w = static_cast<char*>(ws);
+    // ws: [Y-line copy | Y-partial image] (layout of projection / projection_t) followed by the bin tables
     const WsLayout la = layout(p, CTLIB_OP_PRJ, batch, dtype);
     const WsLayout lb = layout(p, CTLIB_OP_PRJ_T, batch, dtype);
     const size_t base = la.total > lb.total ? la.total : lb.total;
-    void* ybuf = y ? y : static_cast<void*>(w + base);
-    hipError_t e = run_to_sino(p, dt, x, ybuf, B, false, P.post_prj, la, w, s);
-    double2* tabs = reinterpret_cast<double2*>(w + lb.tables);
-    if (e == hipSuccess) e = run_tables(p, dt, ybuf, m, P.post_prj, tabs, B, s);
+    // forward fused with the residual tables, then the adjoint (no separate table pass)
+    double2* tabs = reinterpret_cast<double2*>(w + base);
+    void* yl = w + la.ylines;
+    hipError_t e = hipSuccess;
+    if (P.ny) {
+        KernelTimer t(K_AUX, 0, 0, s);
+        e = launch_to_ylines(x, yl, dt, B, P.H, P.W, s);
+    }
+    if (e == hipSuccess) {
+        KernelTimer t(K_FWD, sweep_bytes(P, B, dt), 0, s);
+        e = launch_fwd_resid(P, dt, x, P.ny ? yl : nullptr, y, m, tabs, B, default_fwd_shape(P, false, p->fwd_G), s);
+    }
     if (e == hipSuccess) e = run_to_image(p, dt, tabs, grad, B, false, 1.0, lb, w, s);
     if (e != hipSuccess) return hip_fail(e, "kernel launch");
     return CTLIB_OK;

@@ -1,0 +1,526 @@
+// Device-side building blocks of the distance-driven projector kernels
+// (gfx950).  Each operator is written as a per-work-item block function so the
+// stand-alone kernels (dd_kernels.hip) and the persistent heterogeneous batch
+// kernel (batch.hip) run the same code.
+//
+// Every operator of the family (fan_ed_kernel.cu, fan_ea_kernel.cu,
+// para_kernel.cu) is, per view and per image line, the overlap-weighted
+// resampling of a piecewise-constant function between two sorted edge grids
+// on the sweep axis: the D+1 detector-bin edges e_k and the n+1 pixel edges
+// pi_p = a + s*p of the line.  The reference walks both grids with a
+// data-dependent 4-way merge per thread and scatters with atomics.  Here both
+// directions are evaluated branch-free through FP64 cumulative integrals:
+//
+//   line -> bin (projection, backprojection_t):
+//     F_l(tau) = integral_0^tau line(t) dt  (tau in pixel units), stored as a
+//     table {C_j, slope_j} so that F_l(tau) = C_j + tau*slope_j with
+//     j = trunc(tau + 1) clamped to [0, n+1].  Each thread owns a few bin edges
+//     and accumulates  S_k = sum_l s_l * F_l(tau_l(e_k));  the bin value is
+//     post_k * (S_{k+1} - S_k).  One table per line is shared by G views.
+//
+//   bin -> pixel (projection_t, backprojection):
+//     G_v(e) = integral of the per-bin values over the axis, stored per view as
+//     {C_j, g_j} with G_v(e) = C_j + e*g_j and j found by a closed-form
+//     (Moebius / arctangent / affine) axis -> bin map.  Each lane owns pixel
+//     edges; projection_t accumulates T_p = sum_v G_v(pi_p) and the pixel value
+//     is T_{p+1} - T_p; backprojection weights each per-view difference.
+//
+// Accumulation is FP64 throughout (FP64 FMA issues at the FP32 rate on CDNA4);
+// only the image / sinogram storage is fp32 (or fp64).  No atomics: every
+// output element is written once, so results are bitwise deterministic.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "kernels.h"
+
+namespace ctlib {
+namespace dd {
+
+constexpr int NT = 256;      // threads per block (4 wave64)
+constexpr int LC_WORDS = 8;  // doubles per LineCoef: a, s, A, B, w0, w1, w2, inv_s
+enum { LC_A0 = 0, LC_S = 1, LC_A = 2, LC_B = 3, LC_W0 = 4, LC_W1 = 5, LC_W2 = 6, LC_INVS = 7 };
+enum { GEO_ED = 0, GEO_EA = 1, GEO_PARA = 2 };
+
+// Blocks are dispatched round-robin over the 8 XCDs (block id % 8).  Give each
+// XCD a contiguous range of work items so that blocks sharing an image or a
+// sinogram (same batch element) share that XCD's L2.
+__device__ __forceinline__ int xcd_remap(int bid, int nblocks, int nwork) {
+    const int per = nblocks >> 3;  // nblocks is a multiple of 8
+    const int w = (bid & 7) * per + (bid >> 3);
+    return w < nwork ? w : -1;
+}
+
+inline int grid_for(int nwork) { return ((nwork + 7) / 8) * 8; }
+
+// Index of a cumulative table from a float coordinate, clamped to [0, hi].
+__device__ __forceinline__ int table_index(float q, float hi) { return (int)__builtin_amdgcn_fmed3f(q, 0.0f, hi); }
+
+__device__ __forceinline__ double shfl_up_d(double v, int off) { return __shfl_up(v, off, 64); }
+__device__ __forceinline__ double shfl_down_d(double v, int off) { return __shfl_down(v, off, 64); }
+__device__ __forceinline__ double readlane_d(double v, int lane) {
+    const int2 w = *reinterpret_cast<int2*>(&v);
+    int2 r;
+    r.x = __builtin_amdgcn_readlane(w.x, lane);
+    r.y = __builtin_amdgcn_readlane(w.y, lane);
+    return *reinterpret_cast<double*>(&r);
+}
+
+// Inclusive wave64 scan.
+__device__ __forceinline__ double wave_scan_incl(double v, int lane) {
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const double t = shfl_up_d(v, off);
+        if (lane >= off) v += t;
+    }
+    return v;
+}
+
+// ---------------------------------------------------------------------------
+// 32x32 tile transposes for the Y-axis lines (columns read bottom to top)
+// ---------------------------------------------------------------------------
+
+// xT[c][i] = x[H-1-i][c] for the tile at (r0, c0) of one image.
+template <typename T>
+__device__ __forceinline__ void ylines_tile(const T* __restrict__ xb, T* __restrict__ tb, int H, int W, int r0, int c0,
+                                            T (*tile)[33]) {
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+#pragma unroll
+    for (int k = 0; k < 32; k += 8) {
+        const int r = r0 + ty + k, c = c0 + tx;
+        if (r < H && c < W) tile[ty + k][tx] = xb[(size_t)r * W + c];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 32; k += 8) {
+        const int c = c0 + ty + k;   // output line
+        const int r = r0 + 31 - tx;  // row read by this lane
+        if (c < W && r < H) tb[(size_t)c * H + (H - 1 - r)] = tile[31 - tx][ty + k];
+    }
+}
+
+// out[r][c] = (hasX ? out[r][c] : 0) + (hasY ? yt[c][H-1-r] : 0) for the tile at (r0, c0).
+template <typename T>
+__device__ __forceinline__ void combine_tile(T* __restrict__ ob, const T* __restrict__ yb, int H, int W, int r0, int c0,
+                                             int hasX, int hasY, T (*tile)[33]) {
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    if (hasY) {
+#pragma unroll
+        for (int k = 0; k < 32; k += 8) {
+            const int c = c0 + ty + k;
+            const int r = r0 + 31 - tx;
+            if (c < W && r < H) tile[ty + k][31 - tx] = yb[(size_t)c * H + (H - 1 - r)];
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int k = 0; k < 32; k += 8) {
+        const int r = r0 + ty + k, c = c0 + tx;
+        if (r < H && c < W) {
+            T v = hasX ? ob[(size_t)r * W + c] : T(0);
+            if (hasY) v += tile[tx][ty + k];
+            ob[(size_t)r * W + c] = v;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Line -> bin (projection / backprojection_t)
+// ---------------------------------------------------------------------------
+
+// Entry of a line table: F(tau) = c + tau * s, 16 bytes (one ds_read_b128).
+// A 12-byte form (fp32 slope, ds_read_b96) measured 60 % slower on gfx950:
+// the 16-byte-strided b96 reads conflict in LDS banks.
+template <typename T>
+struct __attribute__((aligned(16))) LEnt {
+    double c;
+    double s;
+};
+
+__device__ __forceinline__ void lent_read(const void* p, double& c, double& s) {
+    const double2 v = *reinterpret_cast<const double2*>(p);
+    c = v.x;
+    s = v.y;
+}
+__device__ __forceinline__ void lent_write(void* p, double c, double s) {
+    *reinterpret_cast<double2*>(p) = make_double2(c, s);
+}
+
+// Builds one line table {C_j, slope_j}, j = 0..n+1, from `val` (this thread's
+// pixels p0 .. p0+m-1; already rounded to the storage precision of the slope).
+// `ws` holds 4 wave sums; one barrier inside.
+template <typename T, int M>
+__device__ __forceinline__ void line_table(const double (&val)[M], int m, int p0, int n, LEnt<T>* __restrict__ tb,
+                                           double* __restrict__ ws, int tid, int lane, int wave) {
+    double run = 0;
+#pragma unroll
+    for (int i = 0; i < M; i++)
+        if (i < m) run += val[i];
+    const double inc = wave_scan_incl(run, lane);
+    if (lane == 63) ws[wave] = inc;
+    __syncthreads();
+    double base = 0;
+    for (int q = 0; q < wave; q++) base += ws[q];
+    double pref = base + inc - run;  // sum of all pixels before p0
+#pragma unroll
+    for (int i = 0; i < M; i++) {
+        if (i < m) {
+            const int p = p0 + i;
+            if (p < n) lent_write(tb + p + 1, pref - (double)(p + 1) * val[i], val[i]);
+            pref += val[i];
+        }
+    }
+    if (tid == 0) lent_write(tb, 0.0, 0.0);
+    if (tid == NT - 1) lent_write(tb + n + 1, ws[0] + ws[1] + ws[2] + ws[3], 0.0);
+}
+
+// LDS bytes of one fwd_block with G views per block.
+__host__ __device__ inline size_t fwd_block_lds(int nmax, int G, bool weighted, int entry_bytes) {
+    const int ntab = weighted ? G : 1;
+    return (size_t)2 * ntab * (nmax + 2) * entry_bytes + (size_t)2 * G * LC_WORDS * 8 + (size_t)2 * ntab * 4 * 8;
+}
+
+// One line -> bin work item: batch element b, sweep axis `mode`, views
+// vlist[grp*G .. grp*G+G-1] of that axis (G = 4*VPW).  `src` holds the lines
+// of that axis.  Wave w owns views w*VPW .. w*VPW+VPW-1 and, for each, the bin
+// edges k = lane + 64*i (i < NEM).  Per image line: the line is loaded one
+// iteration ahead into registers, its cumulative table is built in LDS
+// (double-buffered) by the whole block, and the G views' LineCoefs for the line
+// (prefetched one line ahead by G*8 threads with vector loads) are staged in
+// LDS next to it; each wave then reads only its own views' coefficients.
+// The bin values y_k = post_k (S_{k+1} - S_k) are formed with wave shuffles.
+// RESID: also write the residual bin tables of projection_t (g = (y - m) post).
+template <typename T, int VPW, int NEM, int M, bool WEIGHTED, bool RESID>
+__device__ __forceinline__ void fwd_block(const DevPlan& P, const T* __restrict__ src, T* __restrict__ y, int b,
+                                          int mode, int grp, const double* __restrict__ post,
+                                          const T* __restrict__ m, double2* __restrict__ tabs,
+                                          unsigned char* __restrict__ smem) {
+    constexpr int G = 4 * VPW;
+    constexpr int NTAB = WEIGHTED ? G : 1;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int* vlist = mode == MODE_Y ? P.yviews : P.xviews;
+    const int nv = mode == MODE_Y ? P.ny : P.nx;
+    const int nlines = mode == MODE_Y ? P.W : P.H;
+    const int n = mode == MODE_Y ? P.H : P.W;
+    const int D = P.D;
+    const int ne = (D + 1 + 63) / 64;  // edge slots in use (<= NEM)
+
+    int vw[VPW];
+#pragma unroll
+    for (int v = 0; v < VPW; v++) {
+        const int idx = grp * G + wave * VPW + v;
+        vw[v] = vlist[idx < nv ? idx : nv - 1];
+    }
+    double e[VPW][NEM], S[VPW][NEM];
+#pragma unroll
+    for (int v = 0; v < VPW; v++) {
+#pragma unroll
+        for (int i = 0; i < NEM; i++) {
+            const int k = lane + 64 * i;
+            e[v][i] = P.edges[(size_t)vw[v] * (D + 1) + (k <= D ? k : D)];
+            S[v][i] = 0.0;
+        }
+    }
+
+    const int tabn = n + 2;
+    LEnt<T>* tab = reinterpret_cast<LEnt<T>*>(smem);                // [2][NTAB][tabn]
+    double* cf = reinterpret_cast<double*>(tab + 2 * NTAB * tabn);  // [2][G][LC_WORDS]
+    double* wsum = cf + 2 * G * LC_WORDS;                           // [2][NTAB][4]
+    const T* base = src + (size_t)b * nlines * n;
+    const int m_px = (n + NT - 1) / NT;
+    const int p0 = tid * m_px;
+    const float hi = (float)(n + 1);
+
+    // coefficient word `cw` of view `cg` for the next line, held by thread tid < G*8
+    const bool cthread = tid < G * LC_WORDS;
+    const int cg = tid / LC_WORDS, cw = tid % LC_WORDS;
+    const double* crow = nullptr;
+    if (cthread) {
+        const int idx = grp * G + cg;
+        crow = reinterpret_cast<const double*>(P.lines + (size_t)vlist[idx < nv ? idx : nv - 1] * P.max_lines) + cw;
+    }
+    double cnext = cthread ? crow[0] : 0.0;
+
+    T xv[M];
+#pragma unroll
+    for (int i = 0; i < M; i++) xv[i] = (i < m_px && p0 + i < n) ? base[p0 + i] : T(0);
+
+    for (int l = 0; l < nlines; l++) {
+        LEnt<T>* tb = tab + (l & 1) * NTAB * tabn;
+        double* ws = wsum + (l & 1) * NTAB * 4;
+        double* cfl = cf + (l & 1) * G * LC_WORDS;
+        if (cthread) cfl[tid] = cnext;
+        if (WEIGHTED) __syncthreads();  // the weights below read cfl
+#pragma unroll
+        for (int t = 0; t < NTAB; t++) {
+            double val[M];
+            if (WEIGHTED) {
+                const double w0 = cfl[t * LC_WORDS + LC_W0], w1 = cfl[t * LC_WORDS + LC_W1];
+                const double w2 = cfl[t * LC_WORDS + LC_W2], is = cfl[t * LC_WORDS + LC_INVS];
+#pragma unroll
+                for (int i = 0; i < M; i++) {
+                    val[i] = 0.0;
+                    if (i < m_px && p0 + i < n) {
+                        const double u = __builtin_fma(w1, (double)(p0 + i), w0);
+                        const double d = __builtin_fma(u, u, w2);
+                        // rounded to the slope's storage precision before the prefix sum
+                        val[i] = (double)(T)((double)xv[i] * (is * (double)__builtin_amdgcn_rcpf((float)d)));
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int i = 0; i < M; i++) val[i] = (i < m_px && p0 + i < n) ? (double)xv[i] : 0.0;
+            }
+            line_table<T, M>(val, m_px, p0, n, tb + t * tabn, ws + t * 4, tid, lane, wave);
+        }
+        if (l + 1 < nlines) {
+            const T* nb = base + (size_t)(l + 1) * n;
+#pragma unroll
+            for (int i = 0; i < M; i++) xv[i] = (i < m_px && p0 + i < n) ? nb[p0 + i] : T(0);
+            if (cthread) cnext = crow[(size_t)(l + 1) * LC_WORDS];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int v = 0; v < VPW; v++) {
+            const int g = wave * VPW + v;
+            const double s = cfl[g * LC_WORDS + LC_S];
+            const double A = cfl[g * LC_WORDS + LC_A];
+            const double Bc = cfl[g * LC_WORDS + LC_B];
+            const LEnt<T>* t = tb + (WEIGHTED ? g : 0) * tabn;
+            // every slot is evaluated (slots past the last edge repeat edge D): no branch
+            // between the independent evaluations, so their LDS reads overlap
+#pragma unroll
+            for (int i = 0; i < NEM; i++) {
+                const double tau = __builtin_fma(A, e[v][i], Bc);
+                double c, sl;
+                lent_read(t + table_index((float)tau, hi), c, sl);
+                S[v][i] = __builtin_fma(s, __builtin_fma(tau, sl, c), S[v][i]);
+            }
+        }
+    }
+
+    // Bin values from neighbouring edge sums, per wave.
+#pragma unroll
+    for (int v = 0; v < VPW; v++) {
+        const int idx = grp * G + wave * VPW + v;
+        if (idx >= nv) continue;  // wave-uniform
+        const int view = vw[v];
+        const bool rev = P.vp[view].rev;
+        const size_t row = (size_t)b * P.V + view;
+        const double* pr = post ? post + (size_t)view * D : nullptr;
+        const double* ev = P.edges + (size_t)view * (D + 1);
+        double2* tb = RESID ? tabs + row * (D + 2) : nullptr;
+        double carry = 0.0;
+#pragma unroll
+        for (int i = 0; i < NEM; i++) {
+            if (i >= ne) continue;  // wave-uniform
+            const int k = lane + 64 * i;
+            double nxt = shfl_down_d(S[v][i], 1);
+            double r0 = 0.0;
+#pragma unroll
+            for (int q = 0; q < NEM; q++)
+                if (q == i + 1) r0 = readlane_d(S[v][q], 0);
+            if (lane == 63) nxt = r0;
+            double yk = nxt - S[v][i];
+            if (pr && k < D) yk *= pr[k];
+            const int o = rev ? D - 1 - k : k;
+            if (!RESID) {
+                if (k < D) y[row * D + o] = (T)yk;
+            } else {
+                double gk = 0.0, ek = 0.0, wg = 0.0;
+                if (k < D) {
+                    if (y) y[row * D + o] = (T)yk;
+                    gk = (yk - (double)m[row * D + o]) * pr[k];
+                    ek = ev[k];
+                    wg = (ev[k + 1] - ek) * gk;
+                }
+                const double inc = wave_scan_incl(wg, lane);
+                if (k < D) tb[k + 1] = make_double2(carry + (inc - wg) - ek * gk, gk);
+                carry += readlane_d(inc, 63);
+            }
+        }
+        if (RESID && lane == 0) {
+            tb[0] = make_double2(0.0, 0.0);
+            tb[D + 1] = make_double2(carry, 0.0);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Bin -> pixel (projection_t / backprojection)
+// ---------------------------------------------------------------------------
+
+template <int GEO>
+__device__ __forceinline__ double eval_G(const double2* __restrict__ t, double a, double s, int p, const float (&lk)[4],
+                                         float ea_sc, float ea_off, float hi) {
+    const double pi = __builtin_fma(s, (double)p, a);
+    const float pf = (float)pi;
+    float q;
+    if (GEO == GEO_PARA) {
+        q = __builtin_fmaf(lk[0], pf, lk[1]);
+    } else {
+        const float num = __builtin_fmaf(lk[0], pf, lk[1]);
+        const float den = __builtin_fmaf(lk[2], pf, lk[3]);
+        q = num * __builtin_amdgcn_rcpf(den);
+        if (GEO == GEO_EA) q = __builtin_fmaf(atanf(q), ea_sc, ea_off);
+    }
+    const double2 c = t[table_index(q, hi)];
+    return __builtin_fma(pi, c.y, c.x);
+}
+
+// LDS bytes of one adj_block.
+__host__ __device__ inline size_t adj_block_lds(int D, int L) {
+    return (size_t)2 * (D + 2) * 16 + (size_t)2 * L * LC_WORDS * 8;
+}
+
+// One bin -> pixel work item: batch element b, sweep axis `mode`, image lines
+// [tile*L, tile*L + L) (L = 4*LPW; rows for X, columns bottom-to-top for Y)
+// over all views of that axis.  X lines are written to outX [B][H][W], Y lines
+// to outYT [B][W][H].  Per view: the next view's bin table is copied
+// global -> LDS with global_load_lds_dwordx4 (asynchronous, no register
+// staging) and the next view's LineCoefs of the tile are prefetched into one
+// register per thread while the current view is evaluated; one barrier per view.
+template <typename T, int LPW, int NPE, bool WEIGHTED, int GEO>
+__device__ __forceinline__ void adj_block(const DevPlan& P, const double2* __restrict__ tabs, T* __restrict__ outX,
+                                          T* __restrict__ outYT, int b, int mode, int tile, double wscale,
+                                          unsigned char* __restrict__ smem) {
+    constexpr int L = 4 * LPW;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int* vlist = mode == MODE_Y ? P.yviews : P.xviews;
+    const int nv = mode == MODE_Y ? P.ny : P.nx;
+    const int nlines = mode == MODE_Y ? P.W : P.H;
+    const int n = mode == MODE_Y ? P.H : P.W;
+    const int D = P.D;
+    const int npe = (n + 1 + 63) / 64;  // edge groups (<= NPE)
+    const float hi = (float)(D + 1);
+    const int tabn = D + 2;
+
+    double2* stab = reinterpret_cast<double2*>(smem);          // [2][tabn]
+    double* scf = reinterpret_cast<double*>(stab + 2 * tabn);  // [2][L][LC_WORDS]
+
+    const double2* tb_b = tabs + (size_t)b * P.V * tabn;
+    // Asynchronous copy of view v's table into LDS buffer `buf`: wave w copies
+    // entries [64*(w + 4r), 64*(w + 4r) + 64); lanes past the end are masked off.
+    auto table_to_lds = [&](int v, int buf) {
+        const double2* src = tb_b + (size_t)v * tabn;
+        for (int j0 = wave * 64; j0 < tabn; j0 += NT) {
+            if (j0 + lane < tabn)
+                __builtin_amdgcn_global_load_lds((const void*)(src + j0 + lane),
+                                                 (__attribute__((address_space(3))) void*)(stab + buf * tabn + j0), 16,
+                                                 0, 0);
+        }
+    };
+    // LineCoef word `tid % 8` of tile line `tid / 8`, held by thread tid < L*8.
+    const bool cthread = tid < L * LC_WORDS;
+    const int cl = tile * L + tid / LC_WORDS;
+    const bool cvalid = cthread && cl < nlines;
+    auto coef_word = [&](int v) {
+        return cvalid ? reinterpret_cast<const double*>(P.lines + (size_t)v * P.max_lines + cl)[tid % LC_WORDS] : 0.0;
+    };
+
+    double acc[LPW][NPE];
+#pragma unroll
+    for (int q = 0; q < LPW; q++)
+#pragma unroll
+        for (int i = 0; i < NPE; i++) acc[q][i] = 0.0;
+
+    table_to_lds(vlist[0], 0);
+    if (cthread) scf[tid] = coef_word(vlist[0]);
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    __syncthreads();
+
+    for (int iv = 0; iv < nv; iv++) {
+        const int cur = iv & 1;
+        const int v = vlist[iv];
+        double cnext = 0.0;
+        if (iv + 1 < nv) {
+            const int vn = vlist[iv + 1];
+            table_to_lds(vn, cur ^ 1);
+            cnext = coef_word(vn);
+        }
+        const double2* t = stab + cur * tabn;
+        const double* cf = scf + cur * L * LC_WORDS;
+        const ViewParams& vp = P.vp[v];
+        const float lk[4] = {vp.lk[0], vp.lk[1], vp.lk[2], vp.lk[3]};
+        const float ea_sc = vp.ea_sc, ea_off = vp.ea_off;
+#pragma unroll
+        for (int q = 0; q < LPW; q++) {
+            const int lq = q * 4 + wave;  // line within the tile
+            const int l = tile * L + lq;
+            if (l >= nlines) continue;    // wave-uniform
+            const double a = cf[lq * LC_WORDS + LC_A0], s = cf[lq * LC_WORDS + LC_S];
+            if (!WEIGHTED) {
+#pragma unroll
+                for (int i = 0; i < NPE; i++)
+                    if (i < npe) acc[q][i] += eval_G<GEO>(t, a, s, lane + 64 * i, lk, ea_sc, ea_off, hi);
+            } else {
+                const double w0 = cf[lq * LC_WORDS + LC_W0], w1 = cf[lq * LC_WORDS + LC_W1];
+                const double w2 = cf[lq * LC_WORDS + LC_W2];
+                const double ws = cf[lq * LC_WORDS + LC_INVS] * wscale;
+                auto wgt = [&](int p) {
+                    const double u = __builtin_fma(w1, (double)p, w0);
+                    const double d = __builtin_fma(u, u, w2);
+                    return ws * (double)__builtin_amdgcn_rcpf((float)d);
+                };
+                double gprev = 0.0;
+#pragma unroll
+                for (int i = 0; i < NPE; i++) {
+                    if (i < npe) {
+                        const double gi = eval_G<GEO>(t, a, s, lane + 64 * i, lk, ea_sc, ea_off, hi);
+                        if (i > 0) {
+                            double right = shfl_down_d(gprev, 1);
+                            const double r0 = readlane_d(gi, 0);
+                            if (lane == 63) right = r0;
+                            const int p = lane + 64 * (i - 1);
+                            if (p < n) acc[q][i - 1] += wgt(p) * (right - gprev);
+                        }
+                        gprev = gi;
+                    }
+                }
+                // pixels of the last edge group (their right edges are in the same group)
+                {
+                    const double right = shfl_down_d(gprev, 1);
+                    const int p = lane + 64 * (npe - 1);
+                    if (p < n) {
+#pragma unroll
+                        for (int i = 0; i < NPE; i++)
+                            if (i == npe - 1) acc[q][i] += wgt(p) * (right - gprev);
+                    }
+                }
+            }
+        }
+        if (iv + 1 < nv && cthread) scf[(cur ^ 1) * L * LC_WORDS + tid] = cnext;
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        __syncthreads();
+    }
+
+    // Write pixel values.
+#pragma unroll
+    for (int q = 0; q < LPW; q++) {
+        const int l = tile * L + q * 4 + wave;
+        if (l >= nlines) continue;
+        T* dst = mode == MODE_Y ? outYT + ((size_t)b * P.W + l) * P.H : outX + ((size_t)b * P.H + l) * P.W;
+#pragma unroll
+        for (int i = 0; i < NPE; i++) {
+            if (i >= npe) continue;  // wave-uniform
+            const int p = lane + 64 * i;
+            double val;
+            if (!WEIGHTED) {
+                double right = shfl_down_d(acc[q][i], 1);
+                double r0 = 0.0;
+#pragma unroll
+                for (int k = 0; k < NPE; k++)
+                    if (k == i + 1) r0 = readlane_d(acc[q][k], 0);
+                if (lane == 63) right = r0;
+                val = right - acc[q][i];
+            } else {
+                val = acc[q][i];
+            }
+            if (p < n) dst[p] = (T)val;
+        }
+    }
+}
+
+}  // namespace dd
+}  // namespace ctlib

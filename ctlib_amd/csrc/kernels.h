@@ -5,6 +5,8 @@
 
 #include "plan.h"
 
+struct ctlib_plan;
+
 namespace ctlib {
 
 // Device pointers into one plan allocation (all read-only after upload).
@@ -40,6 +42,11 @@ hipError_t launch_fwd(const DevPlan& P, int dtype, const void* srcx, const void*
                       bool weighted, const double* post, FwdShape shape, hipStream_t s);
 size_t fwd_lds_bytes(const DevPlan& P, bool weighted, int G);
 
+// projection fused with the residual bin tables of projection_t: writes y = A x (y may be null)
+// and tabs[b][v] = tables of g = (y - m) * post_prj, so that launch_adj yields A^T (A x - m).
+hipError_t launch_fwd_resid(const DevPlan& P, int dtype, const void* srcx, const void* srcy, void* y, const void* m,
+                            double2* tabs, int B, FwdShape shape, hipStream_t s);
+
 // Per (b, view) cumulative-integral tables {C_j, g_j}, j = 0..D+1, of g_k = (y - m)[o(k)] * pre[k] * scale.
 hipError_t launch_bin_tables(const DevPlan& P, int in_dtype, const void* y, const void* m, const double* pre,
                              double scale, double2* tabs, int B, hipStream_t s);
@@ -58,5 +65,8 @@ hipError_t launch_combine(void* out, const void* yt, int dtype, int B, int H, in
 // with h the RL taps of the plan.  in/out dtypes are DT_F32 or DT_F64.
 hipError_t launch_ramp(const DevPlan& P, int in_dtype, const void* in, int out_dtype, void* out, long rows,
                        const double* pre, const double* post, double scale, hipStream_t s);
+
+// Device view of a plan (for the batch module).
+DevPlan plan_dev(const struct ::ctlib_plan* plan);
 
 }  // namespace ctlib

@@ -211,6 +211,12 @@ int64_t workspace_bytes(torch::Tensor options, int64_t op, int64_t batch, bool f
     return (int64_t)ctlib_workspace_bytes(c.plan.get(), (int)op, batch, f64 ? CTLIB_F64 : CTLIB_F32);
 }
 
+void set_plan_tuning(torch::Tensor options, int64_t device, int64_t fwd_views_per_block) {
+    Cached c = get_plan(options, (int)device);
+    int rc = ctlib_plan_set_tuning(c.plan.get(), (int)fwd_views_per_block);
+    if (rc) raise_status(rc, "set_plan_tuning");
+}
+
 void timing_enable(bool on) { ctlib_timing_enable(on ? 1 : 0); }
 
 // Per kernel class: launches, total milliseconds, algorithmic bytes and flops of the
@@ -232,6 +238,89 @@ pybind11::dict timing_collect() {
     return d;
 }
 
+// Heterogeneous batch (see ctlib_batch_* in the C ABI).
+class Batch {
+  public:
+    Batch(std::vector<torch::Tensor> options, int64_t device, bool f64) : device_((int)device) {
+        std::vector<ctlib_geom> geoms;
+        for (auto& o : options) {
+            std::vector<double> vals = options_values(o);
+            ctlib_geom g{};
+            int rc = ctlib_parse_options(vals.data(), (int)vals.size(), &g);
+            if (rc) raise_status(rc, "options");
+            geoms.push_back(g);
+        }
+        geoms_ = geoms;
+        c10::hip::HIPGuard guard((c10::DeviceIndex)device_);
+        int rc = ctlib_batch_create(geoms.data(), (int)geoms.size(), device_, f64 ? CTLIB_F64 : CTLIB_F32, &b_);
+        if (rc) throw std::runtime_error(std::string("ctlib_batch_create: ") + ctlib_batch_last_error());
+    }
+    ~Batch() { ctlib_batch_destroy(b_); }
+    int64_t workspace_bytes() const { return (int64_t)ctlib_batch_workspace_bytes(b_); }
+    void set_schedule(int64_t blocks_per_cu, int64_t order) {
+        int rc = ctlib_batch_set_schedule(b_, (int)blocks_per_cu, (int)order);
+        if (rc) throw pybind11::value_error(ctlib_batch_last_error());
+    }
+    void bind(std::vector<torch::Tensor> x, std::vector<torch::Tensor> m, std::vector<torch::Tensor> y,
+              std::vector<torch::Tensor> grad, torch::Tensor ws) {
+        const size_t n = geoms_.size();
+        if (x.size() != n || m.size() != n || grad.size() != n || (!y.empty() && y.size() != n))
+            throw pybind11::value_error("bind needs one tensor per slice");
+        std::vector<const void*> px(n), pm(n);
+        std::vector<void*> py(n), pg(n);
+        for (size_t s = 0; s < n; s++) {
+            const ctlib_geom& g = geoms_[s];
+            auto chk = [&](const torch::Tensor& t, int64_t r, int64_t c, const char* what) {
+                TORCH_CHECK(t.is_cuda() && t.is_contiguous() && t.get_device() == device_, what,
+                            " must be a contiguous tensor on the batch's device");
+                TORCH_CHECK(t.numel() == r * c, what, " of slice ", s, " must hold ", r, " x ", c, " elements");
+            };
+            chk(x[s], g.height, g.width, "x");
+            chk(m[s], g.views, g.dets, "m");
+            chk(grad[s], g.height, g.width, "grad");
+            px[s] = x[s].data_ptr();
+            pm[s] = m[s].data_ptr();
+            pg[s] = grad[s].data_ptr();
+            if (!y.empty()) {
+                chk(y[s], g.views, g.dets, "y");
+                py[s] = y[s].data_ptr();
+            }
+        }
+        TORCH_CHECK(ws.is_cuda() && ws.get_device() == device_, "workspace must be on the batch's device");
+        c10::hip::HIPGuard guard((c10::DeviceIndex)device_);
+        int rc = ctlib_batch_bind(b_, px.data(), pm.data(), y.empty() ? nullptr : py.data(), pg.data(), ws.data_ptr(),
+                                  (size_t)ws.numel() * ws.element_size());
+        if (rc) throw std::runtime_error(std::string("ctlib_batch_bind: ") + ctlib_batch_last_error());
+    }
+    void run() {
+        c10::hip::HIPGuard guard((c10::DeviceIndex)device_);
+        hipStream_t s = c10::hip::getCurrentHIPStream((c10::DeviceIndex)device_).stream();
+        int rc = ctlib_batch_run(b_, (void*)s);
+        if (rc) throw std::runtime_error(std::string("ctlib_batch_run: ") + ctlib_batch_last_error());
+    }
+    void check() {
+        c10::hip::HIPGuard guard((c10::DeviceIndex)device_);
+        hipStream_t s = c10::hip::getCurrentHIPStream((c10::DeviceIndex)device_).stream();
+        int rc = ctlib_batch_check(b_, (void*)s);
+        if (rc) throw std::runtime_error(std::string("ctlib_batch_check: ") + ctlib_batch_last_error());
+    }
+    pybind11::dict info() const {
+        int nitems = 0, bpc = 0, order = 0;
+        ctlib_batch_info(b_, &nitems, &bpc, &order);
+        pybind11::dict d;
+        d["items"] = nitems;
+        d["blocks_per_cu"] = bpc;
+        d["order"] = order;
+        d["slices"] = (int)geoms_.size();
+        return d;
+    }
+
+  private:
+    int device_;
+    std::vector<ctlib_geom> geoms_;
+    ctlib_batch* b_ = nullptr;
+};
+
 void clear_plan_cache() {
     std::lock_guard<std::mutex> lock(g_mu);
     g_cache.clear();
@@ -251,6 +340,15 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m) {
     m.def("parse_options", &parse_options, "Parse an options vector");
     m.def("workspace_bytes", &workspace_bytes, "Scratch bytes of one op");
     m.def("clear_plan_cache", &clear_plan_cache, "Drop all cached geometry plans");
+    pybind11::class_<Batch>(m, "Batch")
+        .def(pybind11::init<std::vector<torch::Tensor>, int64_t, bool>())
+        .def("workspace_bytes", &Batch::workspace_bytes)
+        .def("set_schedule", &Batch::set_schedule)
+        .def("bind", &Batch::bind)
+        .def("run", &Batch::run)
+        .def("check", &Batch::check)
+        .def("info", &Batch::info);
+    m.def("set_plan_tuning", &set_plan_tuning, "Override the line->bin views per block of a cached plan");
     m.def("timing_enable", &timing_enable, "Record HIP events around the library's kernel launches");
     m.def("timing_collect", &timing_collect, "Per-class kernel times since the last collect");
     m.attr("OP_PRJ") = (int)CTLIB_OP_PRJ;

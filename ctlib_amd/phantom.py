@@ -62,3 +62,30 @@ CONFIGS = {
     2: dict(options=fan_options(1024, 888, 512, 512, 0.0066, 0.0011, 5.95, 4.906, 1), batch=16, scale=0.02),
     4: dict(options=fan_options(1440, 1536, 1024, 1024, 0.0033, 0.0036, 5.95, 4.906, 0), batch=4, scale=0.02),
 }
+
+
+def hetero_slices(n=64, seed=3999):
+    """Heterogeneous batch: per slice an image size from {256, 384, 512}^2, views U[96, 1152],
+    bins from {512, 736, 1024} and scan_type from {0, 1, 2}.  Fan slices use the fan-ED / fan-EA
+    geometries above (source 5.95, detector 4.906, same field of view and detector extent), parallel
+    slices the parallel one; pixel and bin spacings scale with the counts.  Returns
+    (options list, phantom list) with phantoms seeded 3000 + slice."""
+    rng = np.random.default_rng(seed)
+    opts, imgs = [], []
+    for s in range(n):
+        size = int(rng.choice([256, 384, 512]))
+        views = int(rng.integers(96, 1153))
+        dets = int(rng.choice([512, 736, 1024]))
+        st = int(rng.choice([0, 1, 2]))
+        if st == 0:
+            o = fan_options(views, dets, size, size, 0.0066 * 512 / size, 0.0072 * 736 / dets, 5.95, 4.906, 0)
+            scale = 0.02
+        elif st == 1:
+            o = fan_options(views, dets, size, size, 0.0066 * 512 / size, 0.0011 * 888 / dets, 5.95, 4.906, 1)
+            scale = 0.02
+        else:
+            o = para_options(views, dets, size, size, 1.0 / size, (367 / 256) / dets)
+            scale = 1.0
+        opts.append(o)
+        imgs.append(ellipse_phantom(size, size, 3000 + s, scale=scale))
+    return opts, imgs

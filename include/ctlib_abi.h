@@ -90,6 +90,11 @@ int ctlib_run_residual(const ctlib_plan* plan, int dtype, const void* x, const v
                        void* grad, int64_t batch, void* ws, size_t ws_bytes, void* stream);
 size_t ctlib_residual_workspace_bytes(const ctlib_plan* plan, int64_t batch, int dtype);
 
+/* Launch-shape override chosen by the autotuner: views per block of the
+ * line->bin kernel (4 or 8; 0 = automatic; 8 applies when D < 768).  Set it before the plan is
+ * used from several threads; it applies to later ctlib_run calls. */
+int ctlib_plan_set_tuning(ctlib_plan* plan, int fwd_views_per_block);
+
 /* Geometry queries for host code that mirrors the plan (tests, autograd). */
 int ctlib_plan_geom(const ctlib_plan* plan, ctlib_geom* out);
 int ctlib_plan_view_info(const ctlib_plan* plan, int view, int* xmode, int* rev);
@@ -101,6 +106,32 @@ int ctlib_plan_view_info(const ctlib_plan* plan, int view, int* xmode, int* rev)
  * view list int32, 6 Y-axis view list int32, 7 RL taps [2D-1] f64, 8 fbp
  * pre-weight [D] f64.  With dst == NULL only *bytes is set. */
 int ctlib_plan_export(const ctlib_geom* geom, int table, void* dst, size_t* bytes);
+
+/* Heterogeneous batch: slices of different geometry (image size, views, bins,
+ * scan_type), each with one image x_s and one measured sinogram m_s, run as
+ * ONE persistent fused launch computing y_s = A_s x_s and
+ * grad_s = A_s^T (A_s x_s - m_s) for every slice.  Work items (line copies,
+ * projection groups, projection_t tiles, axis combines) are dequeued from one
+ * atomic counter; per-slice counters order the phases.  ctlib_batch_run
+ * allocates nothing, synchronises nothing and records no memset, so it may be
+ * captured in a HIP graph.  Slices: width, height <= 512, dets <= 1279.
+ * No reference counterpart (the reference runs one geometry per call). */
+typedef struct ctlib_batch ctlib_batch;
+int ctlib_batch_create(const ctlib_geom* geoms, int nslices, int device, int dtype, ctlib_batch** out);
+void ctlib_batch_destroy(ctlib_batch* batch);
+size_t ctlib_batch_workspace_bytes(const ctlib_batch* batch);
+/* Schedule knobs chosen by the autotuner: resident blocks per CU (1..8) and
+ * item order within a phase (0 longest first, 1 slice order). */
+int ctlib_batch_set_schedule(ctlib_batch* batch, int blocks_per_cu, int order);
+/* Bind per-slice device pointers (x, m, grad required; y optional) and the
+ * workspace; uploads the slice records (synchronous).  Rebind when any changes. */
+int ctlib_batch_bind(ctlib_batch* batch, const void* const* x, const void* const* m, void* const* y,
+                     void* const* grad, void* ws, size_t ws_bytes);
+int ctlib_batch_run(const ctlib_batch* batch, void* stream);
+int ctlib_batch_info(const ctlib_batch* batch, int* nitems, int* blocks_per_cu, int* order);
+/* Waits for `stream` and reports whether the last run flagged a dependency-wait timeout. */
+int ctlib_batch_check(const ctlib_batch* batch, void* stream);
+const char* ctlib_batch_last_error(void);
 
 /* Kernel timing.  While enabled, ctlib_run / ctlib_run_residual record a HIP
  * event pair on the launch stream around each launch, grouped in classes:
