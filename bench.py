@@ -122,6 +122,52 @@ def workload_hetero(args):
     }
 
 
+def workload_sv(args):
+    """backprojection_sv (per-view backprojection) on fan-ED 512^2, 736 views x 736 bins, B=8.
+
+    Output B*V*H*W fp32 = 6.17 GB per call: a store-bound kernel.  roofline.achieved is its
+    algorithmic bytes 4*B*V*(H*W + D) (images written + sinogram rows read) over the mean
+    launch time of the per-view kernel (HIP events on its stream)."""
+    import torch
+
+    import ctlib_amd
+    from ctlib_amd import _C, phantom
+
+    dev = torch.device("cuda:0")
+    B = 8
+    o = workload_options()
+    ot = torch.from_numpy(o)
+    x = torch.from_numpy(phantom.phantom_batch(B, 512, 512, config=1, scale=0.02)).to(dev)
+    y = ctlib_amd.projection(x, ot)
+    for _ in range(args.warmup):
+        out = ctlib_amd.backprojection_sv(y, ot)
+    del out
+    torch.cuda.synchronize()
+    _C.timing_collect()
+    _C.timing_enable(True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        out = ctlib_amd.backprojection_sv(y, ot)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1000.0 / args.steps
+    _C.timing_enable(False)
+    st = _C.timing_collect()["sv"]
+    k_ms = st["ms"] / max(st["launches"], 1)
+    k_bytes = st["algo_bytes"] / max(st["launches"], 1)
+    achieved = k_bytes / (k_ms * 1e-3) / 1e9
+    return {
+        "metric": "backprojection_sv calls/s, fan-ED 512^2 736x736 B=8 -> [8,736,512,512]",
+        "value": round(1000.0 / ms, 3), "unit": "calls/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "fan-ED 512x512, 736 views x 736 dets, B=8: per-view backprojection",
+                   "global_batch": B, "output_bytes": B * 736 * 512 * 512 * 4},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "k_bprj_sv", "kernel_ms": round(k_ms, 4), "algo_bytes_per_launch": k_bytes},
+    }
+
+
 def workload_unroll(args):
     """50-iteration unrolled gradient descent with learnable step sizes, per-iteration checkpointing."""
     import torch
@@ -197,7 +243,7 @@ def main():
     ap.add_argument("--paths", default="plain,autograd,graph")
     ap.add_argument("--calib-copy", action="store_true",
                     help="run one 256 MiB device copy first (reference kernel for HBM counter scaling)")
-    ap.add_argument("--workload", default="metric", choices=("metric", "hetero", "unroll"),
+    ap.add_argument("--workload", default="metric", choices=("metric", "hetero", "unroll", "sv"),
                     help="metric: the headline fan-ED B=32 step (default); hetero: 64-slice batch; "
                          "unroll: 50-iteration checkpointed reconstruction")
     args = ap.parse_args()
@@ -210,6 +256,9 @@ def main():
         return
     if args.workload == "unroll":
         print(json.dumps(workload_unroll(args)), flush=True)
+        return
+    if args.workload == "sv":
+        print(json.dumps(workload_sv(args)), flush=True)
         return
 
     import torch

@@ -8,6 +8,7 @@ Drop-in for the reference module `ctlib` (ctlib.cpp:106-114):
     img  = ctlib.backprojection(sino, options)       # pixel-normalised, distance-weighted
     sino = ctlib.backprojection_t(image, options)    # transpose of backprojection (reference form)
     img  = ctlib.fbp(sino, options)                  # RL-filtered backprojection
+    imgs = ctlib.backprojection_sv(sino, options)    # per-view backprojection, B x V x H x W (fan-ED)
 
 `options` is the reference's vector: 12 entries for fan beam
 [views, dets, width, height, dImg, dDet, Ang0, dAng, s2r, d2r, binshift, scan_type]
@@ -85,8 +86,15 @@ def parse_options(options):
 
 
 def backprojection_sv(projection, options):
-    """Per-view fan-ED backprojection (ctlib.cpp:45-54): not built yet."""
-    raise NotImplementedError("backprojection_sv is not implemented in ctlib_amd yet")
+    """Per-view backprojection (ctlib.cpp:45-54, fan_ed_kernel.cu:945-1161): B x 1 x V x D -> B x V x H x W.
+
+    Image v holds view v's term of `backprojection`, so the sum over dim 1 equals
+    `backprojection(projection, options)[:, 0]`.  Like the reference, only
+    scan_type 0 (fan beam, equal distance) is accepted; other scan types raise
+    ValueError (the reference exits the process).  The output is B*V*H*W values:
+    6.2 GB in float32 at B=8, 736 views, 512 x 512.
+    """
+    return _C.backprojection_sv(projection, options)
 
 
 def laplacian(input, k):

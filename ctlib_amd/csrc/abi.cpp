@@ -115,6 +115,9 @@ WsLayout layout(const ctlib_plan* p, int op, int64_t B, int dtype) {
         L.n_ylines = P.ny ? img * es : 0;
         L.n_rows = sino * es;
         break;
+    case CTLIB_OP_BPRJ_SV:
+        L.n_tables = (size_t)B * P.V * (P.D + 2) * sizeof(double2);
+        break;
     default:
         break;
     }
@@ -135,7 +138,7 @@ FwdShape default_fwd_shape(const DevPlan& P, bool weighted, int tuned = 0) {
 }
 
 // ---- kernel timing (HIP event pairs around the main launches, on their stream)
-enum { K_FWD = 0, K_ADJ = 1, K_RAMP = 2, K_AUX = 3 };
+enum { K_FWD = 0, K_ADJ = 1, K_RAMP = 2, K_AUX = 3, K_SV = 4 };
 
 struct TimedLaunch {
     hipEvent_t a, b;
@@ -192,7 +195,7 @@ double sweep_bytes(const DevPlan& P, int B, int dt) {
 int check_run_args(const ctlib_plan* p, int op, int dtype, const void* in, void* out, int64_t B, void* ws,
                    size_t ws_bytes) {
     if (!p) return fail(CTLIB_ERR_INVALID_ARG, "plan is NULL");
-    if (op < CTLIB_OP_PRJ || op > CTLIB_OP_FBP_T) return fail(CTLIB_ERR_INVALID_ARG, "unknown op code");
+    if (op < CTLIB_OP_PRJ || op > CTLIB_OP_BPRJ_SV) return fail(CTLIB_ERR_INVALID_ARG, "unknown op code");
     if (dtype != CTLIB_F32 && dtype != CTLIB_F64) return fail(CTLIB_ERR_UNSUPPORTED_DTYPE, "dtype must be f32 or f64");
     if (B < 0 || B > (1 << 24)) return fail(CTLIB_ERR_INVALID_ARG, "batch out of range");
     if (B > 0 && (!in || !out)) return fail(CTLIB_ERR_INVALID_ARG, "in/out must be device pointers");
@@ -506,6 +509,16 @@ int ctlib_run(const ctlib_plan* p, int op, int dtype, const void* in, void* out,
         void* rows = w + L.rows;
         e = run_to_sino(p, dt, in, rows, B, true, nullptr, L, w, s);
         if (e == hipSuccess) e = run_ramp(p, dt, rows, dt, out, B, nullptr, P.preweight, P.dAng / 2, s);
+        break;
+    }
+    case CTLIB_OP_BPRJ_SV: {
+        // per-view terms of backprojection, one image per (b, view) (fan_ed_kernel.cu:1299-1331)
+        if ((size_t)B * P.V > (size_t)INT32_MAX) return fail(CTLIB_ERR_INVALID_ARG, "batch * views exceeds 2^31");
+        e = run_tables(p, dt, in, nullptr, nullptr, tabs, B, s);
+        if (e == hipSuccess) {
+            KernelTimer t(K_SV, sweep_bytes(P, B, dt), 0, s);
+            e = launch_bprj_sv(P, dt, tabs, out, B, 1.0, s);
+        }
         break;
     }
     default:

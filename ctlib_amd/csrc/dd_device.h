@@ -65,13 +65,27 @@ __device__ __forceinline__ double readlane_d(double v, int lane) {
     return *reinterpret_cast<double*>(&r);
 }
 
-// Inclusive wave64 scan.
-__device__ __forceinline__ double wave_scan_incl(double v, int lane) {
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        const double t = shfl_up_d(v, off);
-        if (lane >= off) v += t;
-    }
+// Lane-shifted copy of a double through DPP (both 32-bit halves; lanes the
+// control does not write read 0).
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ double dpp_d(double v) {
+    const int2 w = *reinterpret_cast<int2*>(&v);
+    int2 r;
+    r.x = __builtin_amdgcn_update_dpp(0, w.x, CTRL, ROW_MASK, 0xf, true);
+    r.y = __builtin_amdgcn_update_dpp(0, w.y, CTRL, ROW_MASK, 0xf, true);
+    return *reinterpret_cast<double*>(&r);
+}
+
+// Inclusive wave64 scan without LDS round trips: row_shr 1, 2, 4, 8 scan each
+// 16-lane row, then row_bcast:15 adds row 0's total to row 1 and row 2's to
+// row 3, and row_bcast:31 adds lane 31's running total to rows 2 and 3.
+__device__ __forceinline__ double wave_scan_incl(double v, int) {
+    v += dpp_d<0x111, 0xf>(v);  // row_shr:1
+    v += dpp_d<0x112, 0xf>(v);  // row_shr:2
+    v += dpp_d<0x114, 0xf>(v);  // row_shr:4
+    v += dpp_d<0x118, 0xf>(v);  // row_shr:8
+    v += dpp_d<0x142, 0xa>(v);  // row_bcast:15 -> rows 1, 3
+    v += dpp_d<0x143, 0xc>(v);  // row_bcast:31 -> rows 2, 3
     return v;
 }
 

@@ -106,6 +106,76 @@ __global__ __launch_bounds__(256) void k_combine(T* __restrict__ out, const T* _
                     tile);
 }
 
+
+// backprojection_sv: one view's term of the weighted backprojection, written to
+// its own image, out[b][v][r][c] (fan_ed_kernel.cu:945-1161).  One block per
+// (b, view); the view's bin table is staged in LDS.  X-axis views: a wave owns
+// an image row and its lanes the pixel edges of 64 consecutive columns (the
+// right edge comes from the neighbouring lane), so stores are row-contiguous.
+// Y-axis views: each lane owns a column and walks its pixels bottom to top, so
+// the 64 lanes of a wave store 64 adjacent columns of one row.
+template <typename T, int GEO>
+__global__ __launch_bounds__(256) void k_bprj_sv(DevPlan P, const double2* __restrict__ tabs, T* __restrict__ out,
+                                                 int nitems, double wscale) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int item = xcd_remap(blockIdx.x, gridDim.x, nitems);  // b * V + v
+    if (item < 0) return;
+    const int v = item % P.V;
+    const int D = P.D, H = P.H, W = P.W;
+    const int tabn = D + 2;
+    double2* t = reinterpret_cast<double2*>(smem);
+    const double2* src = tabs + (size_t)item * tabn;
+    for (int j = threadIdx.x; j < tabn; j += NT) t[j] = src[j];
+    __syncthreads();
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const ViewParams& vp = P.vp[v];
+    const float lk[4] = {vp.lk[0], vp.lk[1], vp.lk[2], vp.lk[3]};
+    const float ea_sc = vp.ea_sc, ea_off = vp.ea_off;
+    const float hi = (float)(D + 1);
+    const LineCoef* lc = P.lines + (size_t)v * P.max_lines;
+    T* img = out + (size_t)item * H * W;
+    auto wgt = [&](const LineCoef& c, int p) {
+        const double u = __builtin_fma(c.w1, (double)p, c.w0);
+        const double d = __builtin_fma(u, u, c.w2);
+        return c.inv_s * wscale * (double)__builtin_amdgcn_rcpf((float)d);
+    };
+    if (vp.xmode) {
+        const int npe = (W + 1 + 63) / 64;  // edge groups of a row
+        for (int r = wave; r < H; r += NT / 64) {
+            const LineCoef c = lc[r];
+            T* dst = img + (size_t)r * W;
+            double gprev = 0.0;
+            for (int i = 0; i < npe; i++) {
+                const double gi = eval_G<GEO>(t, c.a, c.s, lane + 64 * i, lk, ea_sc, ea_off, hi);
+                if (i > 0) {
+                    double right = shfl_down_d(gprev, 1);
+                    const double r0 = readlane_d(gi, 0);
+                    if (lane == 63) right = r0;
+                    const int p = lane + 64 * (i - 1);
+                    if (p < W) dst[p] = (T)(wgt(c, p) * (right - gprev));
+                }
+                gprev = gi;
+            }
+            // pixels of the last group: their right edges are in the same group
+            const double right = shfl_down_d(gprev, 1);
+            const int p = lane + 64 * (npe - 1);
+            if (p < W) dst[p] = (T)(wgt(c, p) * (right - gprev));
+        }
+    } else {
+        for (int c0 = wave * 64; c0 < W; c0 += NT) {
+            const int col = c0 + lane;
+            const LineCoef c = lc[col < W ? col : W - 1];
+            double gprev = eval_G<GEO>(t, c.a, c.s, 0, lk, ea_sc, ea_off, hi);
+            T* dst = img + (size_t)(H - 1) * W + col;
+            for (int i = 0; i < H; i++) {
+                const double gi = eval_G<GEO>(t, c.a, c.s, i + 1, lk, ea_sc, ea_off, hi);
+                if (col < W) dst[-(ptrdiff_t)i * W] = (T)(wgt(c, i) * (gi - gprev));
+                gprev = gi;
+            }
+        }
+    }
+}
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -156,12 +226,14 @@ hipError_t fwd_launch_t(const DevPlan& P, const FwdArgs& a, hipStream_t s) {
     return hipGetLastError();
 }
 
+// Pixels per thread of the line table (M >= n / 256).
 template <typename T, int VPW, int NEM, bool WTD, bool RESID>
 hipError_t fwd_m(const DevPlan& P, const FwdArgs& a, hipStream_t s) {
     int n = P.nx ? P.W : 0;
     if (P.ny && P.H > n) n = P.H;
     const int m = (n + NT - 1) / NT;
     if (m <= 2) return fwd_launch_t<T, VPW, NEM, 2, WTD, RESID>(P, a, s);
+    if (m <= 4) return fwd_launch_t<T, VPW, NEM, 4, WTD, RESID>(P, a, s);
     if (m <= 8) return fwd_launch_t<T, VPW, NEM, 8, WTD, RESID>(P, a, s);
     return hipErrorInvalidValue;
 }
@@ -175,7 +247,9 @@ hipError_t fwd_ne(const DevPlan& P, const FwdArgs& a, hipStream_t s) {
     if (ne <= 12) return fwd_m<T, VPW, 12, WTD, RESID>(P, a, s);
     if (VPW != 1) return hipErrorInvalidValue;
     if (ne <= 16) return fwd_m<T, 1, 16, WTD, RESID>(P, a, s);
+    if (ne <= 20) return fwd_m<T, 1, 20, WTD, RESID>(P, a, s);
     if (ne <= 24) return fwd_m<T, 1, 24, WTD, RESID>(P, a, s);
+    if (ne <= 28) return fwd_m<T, 1, 28, WTD, RESID>(P, a, s);
     if (ne <= 32) return fwd_m<T, 1, 32, WTD, RESID>(P, a, s);
     return hipErrorInvalidValue;
 }
@@ -276,6 +350,37 @@ hipError_t launch_adj(const DevPlan& P, int dtype, const double2* tabs, void* ou
     }
     return weighted ? adj_geo<double, true>(P, tabs, outX, outYT, B, wscale, s)
                     : adj_geo<double, false>(P, tabs, outX, outYT, B, wscale, s);
+}
+
+namespace {
+
+template <typename T, int GEO>
+hipError_t sv_launch_t(const DevPlan& P, const double2* tabs, void* out, int B, double wscale, hipStream_t s) {
+    const int nitems = B * P.V;
+    const size_t lds = (size_t)(P.D + 2) * sizeof(double2);
+    hipLaunchKernelGGL((k_bprj_sv<T, GEO>), dim3(grid_for(nitems)), dim3(NT), lds, s, P, tabs, (T*)out, nitems,
+                       wscale);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t sv_geo(const DevPlan& P, const double2* tabs, void* out, int B, double wscale, hipStream_t s) {
+    switch (P.scan_type) {
+    case GEO_ED: return sv_launch_t<T, GEO_ED>(P, tabs, out, B, wscale, s);
+    case GEO_EA: return sv_launch_t<T, GEO_EA>(P, tabs, out, B, wscale, s);
+    default: return sv_launch_t<T, GEO_PARA>(P, tabs, out, B, wscale, s);
+    }
+}
+
+}  // namespace
+
+hipError_t launch_bprj_sv(const DevPlan& P, int dtype, const double2* tabs, void* out, int B, double wscale,
+                          hipStream_t s) {
+    if (B <= 0) return hipSuccess;
+    if ((size_t)B * P.V > (size_t)INT32_MAX || (size_t)(P.D + 2) * sizeof(double2) > 64 * 1024)
+        return hipErrorInvalidValue;
+    if (dtype == DT_F32) return sv_geo<float>(P, tabs, out, B, wscale, s);
+    return sv_geo<double>(P, tabs, out, B, wscale, s);
 }
 
 hipError_t launch_combine(void* out, const void* yt, int dtype, int B, int H, int W, bool hasX, bool hasY,

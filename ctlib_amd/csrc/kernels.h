@@ -56,6 +56,11 @@ hipError_t launch_bin_tables(const DevPlan& P, int in_dtype, const void* y, cons
 hipError_t launch_adj(const DevPlan& P, int dtype, const double2* tabs, void* outX, void* outYT, int B,
                       bool weighted, double wscale, AdjShape shape, hipStream_t s);
 
+// Per-view weighted backprojection (backprojection_sv): out[b][v][H][W] = view v's term of
+// backprojection, pixel weight * wscale, from the bin tables of launch_bin_tables.
+hipError_t launch_bprj_sv(const DevPlan& P, int dtype, const double2* tabs, void* out, int B, double wscale,
+                          hipStream_t s);
+
 // out[b][r][c] = (hasX ? out[b][r][c] : 0) + (hasY ? yt[b][c][H-1-r] : 0)
 hipError_t launch_combine(void* out, const void* yt, int dtype, int B, int H, int W, bool hasX, bool hasY,
                           hipStream_t s);

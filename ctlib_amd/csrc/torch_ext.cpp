@@ -94,7 +94,7 @@ torch::Tensor check_input(const torch::Tensor& x, const char* name) {
     return x;
 }
 
-enum Kind { IMG_TO_SINO, SINO_TO_IMG };
+enum Kind { IMG_TO_SINO, SINO_TO_IMG, SINO_TO_VIEWS };
 
 torch::Tensor run_op(const torch::Tensor& in_, const torch::Tensor& options, int op, Kind kind, const char* name) {
     torch::Tensor in = check_input(in_, name);
@@ -115,8 +115,9 @@ torch::Tensor run_op(const torch::Tensor& in_, const torch::Tensor& options, int
                                         std::to_string(in.size(3)) + " but options give views x dets = " +
                                         std::to_string(g.views) + "x" + std::to_string(g.dets));
     }
-    torch::Tensor out = kind == IMG_TO_SINO ? torch::empty({B, 1, g.views, g.dets}, in.options())
-                                            : torch::empty({B, 1, g.height, g.width}, in.options());
+    torch::Tensor out = kind == IMG_TO_SINO   ? torch::empty({B, 1, g.views, g.dets}, in.options())
+                        : kind == SINO_TO_IMG ? torch::empty({B, 1, g.height, g.width}, in.options())
+                                              : torch::empty({B, g.views, g.height, g.width}, in.options());
     if (B == 0) return out;
     const size_t wsb = ctlib_workspace_bytes(c.plan.get(), op, B, dt);
     torch::Tensor ws;
@@ -147,6 +148,21 @@ torch::Tensor backprojection_t(torch::Tensor image, torch::Tensor options) {
 // ctlib.cpp:86-99
 torch::Tensor fbp(torch::Tensor projection, torch::Tensor options) {
     return run_op(projection, options, CTLIB_OP_FBP, SINO_TO_IMG, "projection");
+}
+// ctlib.cpp:45-54: per-view backprojection, output [B, V, H, W].  The reference
+// implements it for scan_type 0 only and exit(0)s otherwise; that case raises
+// here unless `any_scan_type` asks for the fan-EA / parallel extension.
+torch::Tensor backprojection_sv(torch::Tensor projection, torch::Tensor options, bool any_scan_type) {
+    if (!any_scan_type) {
+        std::vector<double> vals = options_values(options);
+        ctlib_geom g{};
+        int rc = ctlib_parse_options(vals.data(), (int)vals.size(), &g);
+        if (rc) raise_status(rc, "options");
+        if (g.scan_type != 0)
+            throw pybind11::value_error("backprojection_sv supports scan_type 0 (fan beam, equal distance) only, "
+                                        "as the reference does");
+    }
+    return run_op(projection, options, CTLIB_OP_BPRJ_SV, SINO_TO_VIEWS, "projection");
 }
 // Exact adjoints used by autograd (no reference counterpart).
 torch::Tensor backprojection_t_true(torch::Tensor image, torch::Tensor options) {
@@ -220,14 +236,14 @@ void set_plan_tuning(torch::Tensor options, int64_t device, int64_t fwd_views_pe
 void timing_enable(bool on) { ctlib_timing_enable(on ? 1 : 0); }
 
 // Per kernel class: launches, total milliseconds, algorithmic bytes and flops of the
-// launches recorded since the last collect (classes: fwd, adj, ramp, aux).
+// launches recorded since the last collect (classes: fwd, adj, ramp, aux, sv).
 pybind11::dict timing_collect() {
     ctlib_kernel_stats st{};
     int rc = ctlib_timing_collect(&st);
     if (rc) raise_status(rc, "timing_collect");
-    static const char* names[4] = {"fwd", "adj", "ramp", "aux"};
+    static const char* names[5] = {"fwd", "adj", "ramp", "aux", "sv"};
     pybind11::dict d;
-    for (int i = 0; i < 4; i++) {
+    for (int i = 0; i < 5; i++) {
         pybind11::dict e;
         e["launches"] = st.launches[i];
         e["ms"] = st.ms[i];
@@ -336,6 +352,8 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m) {
     m.def("fbp", &fbp, "CT filtered backprojection with RL filter (HIP)");
     m.def("backprojection_t_true", &backprojection_t_true, "Exact adjoint of backprojection (HIP)");
     m.def("fbp_t", &fbp_t, "Exact adjoint of fbp (HIP)");
+    m.def("backprojection_sv", &backprojection_sv, "Per-view backprojection, [B,V,H,W] (HIP)",
+          pybind11::arg("projection"), pybind11::arg("options"), pybind11::arg("any_scan_type") = false);
     m.def("residual_step", &residual_step, "(A x, A^T (A x - m)) in one call (HIP)");
     m.def("parse_options", &parse_options, "Parse an options vector");
     m.def("workspace_bytes", &workspace_bytes, "Scratch bytes of one op");
@@ -358,4 +376,5 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m) {
     m.attr("OP_BPRJ_T_TRUE") = (int)CTLIB_OP_BPRJ_T_TRUE;
     m.attr("OP_FBP") = (int)CTLIB_OP_FBP;
     m.attr("OP_FBP_T") = (int)CTLIB_OP_FBP_T;
+    m.attr("OP_BPRJ_SV") = (int)CTLIB_OP_BPRJ_SV;
 }

@@ -50,7 +50,12 @@ typedef enum ctlib_op {
                                   CTLIB_OP_BPRJ_T only for scan_type 1); no reference
                                   counterpart, used by autograd                           */
     CTLIB_OP_FBP = 5,          /* fbp               sinogram -> image  (ctlib.cpp:86-99) */
-    CTLIB_OP_FBP_T = 6         /* exact adjoint of CTLIB_OP_FBP; no reference counterpart */
+    CTLIB_OP_FBP_T = 6,        /* exact adjoint of CTLIB_OP_FBP; no reference counterpart */
+    CTLIB_OP_BPRJ_SV = 7       /* backprojection_sv sinogram [batch][views][dets] -> per-view
+                                  images [batch][views][height][width]: view v's term of
+                                  CTLIB_OP_BPRJ (ctlib.cpp:45-54, fan_ed_kernel.cu:945-1161,
+                                  1299-1331; the reference has it for scan_type 0 only,
+                                  here every scan_type)                                   */
 } ctlib_op;
 
 /* Geometry as read from the options vector (README.md:56-76). */
@@ -136,15 +141,17 @@ const char* ctlib_batch_last_error(void);
 /* Kernel timing.  While enabled, ctlib_run / ctlib_run_residual record a HIP
  * event pair on the launch stream around each launch, grouped in classes:
  * 0 line->bin DD kernel, 1 bin->pixel DD kernel, 2 ramp GEMM, 3 auxiliary
- * (line transposes, bin tables, axis combine).  ctlib_timing_collect waits for
+ * (line transposes, bin tables, axis combine), 4 per-view backprojection
+ * (CTLIB_OP_BPRJ_SV; its algo_bytes are es*B*V*(H*W + D), the images written
+ * plus the sinogram read); classes 5-7 are unused.  ctlib_timing_collect waits for
  * the recorded events, sums them per class and clears the record.
  * algo_bytes: sweep-model bytes of the DD kernels, es*B*V*(H*W + D) per launch;
  * flops: 2*B*V*D*D per ramp launch.  No reference counterpart. */
 typedef struct ctlib_kernel_stats {
-    int64_t launches[4];
-    double ms[4];
-    double algo_bytes[4];
-    double flops[4];
+    int64_t launches[8];
+    double ms[8];
+    double algo_bytes[8];
+    double flops[8];
 } ctlib_kernel_stats;
 int ctlib_timing_enable(int on);
 int ctlib_timing_collect(ctlib_kernel_stats* out);

@@ -15,6 +15,8 @@
 //       backprojection_t  fan_ed_kernel.cu:501-705  fan_ea_kernel.cu:474-698  para_kernel.cu:338-499
 //       backprojection    fan_ed_kernel.cu:707-923  fan_ea_kernel.cu:700-901  para_kernel.cu:501-663
 //       fbp               fan_ed_kernel.cu:1333-1382 fan_ea_kernel.cu:1059-1107 para_kernel.cu:809-849
+//       backprojection_sv fan_ed_kernel.cu:945-1161, 1299-1331 (per-view images; the
+//                         same merge for fan-EA and parallel beam as an extension)
 //       rlfilter          fan_ed_kernel.cu:925-943  fan_ea_kernel.cu:903-921  para_kernel.cu:665-683
 //       options/dispatch  ctlib.cpp:15-99, fan_ed_kernel.cu:1165-1175, para_kernel.cu:687-695
 //   * ctor_brute  - an O(views*lines*pixels*bins) interval-intersection
@@ -26,7 +28,8 @@
 //   sino   [B][V][D] row-major, bin j spans [(j - D/2), (j + 1 - D/2)] dDet-units
 //   op codes: 0 projection, 1 projection_t, 2 backprojection,
 //             3 backprojection_t (reference form), 4 backprojection_t true adjoint
-//             (differs from 3 only for scan_type 1), 5 fbp.
+//             (differs from 3 only for scan_type 1), 5 fbp,
+//             6 backprojection_sv (sinogram [B][V][D] -> images [B][V][H][W]).
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
@@ -38,7 +41,7 @@
 
 namespace {
 
-enum { OP_PRJ = 0, OP_PRJ_T = 1, OP_BPRJ = 2, OP_BPRJ_T = 3, OP_BPRJ_T_TRUE = 4, OP_FBP = 5 };
+enum { OP_PRJ = 0, OP_PRJ_T = 1, OP_BPRJ = 2, OP_BPRJ_T = 3, OP_BPRJ_T_TRUE = 4, OP_FBP = 5, OP_BPRJ_SV = 6 };
 enum { ST_ED = 0, ST_EA = 1, ST_PARA = 2 };
 enum { E_OK = 0, E_ARG = 1, E_SCAN = 2 };
 
@@ -380,6 +383,28 @@ void run_to_image(const Geom& g, const std::vector<View>& views, const double* i
     }
 }
 
+// Per-view backprojection (backprojection_sv, fan_ed_kernel.cu:945-1161): the merge
+// of backprojection, written to image (b, v) instead of summed over views
+// (fan_ed_kernel.cu:1042,1128 index image[channel][view]).
+void run_sv(const Geom& g, const std::vector<View>& views, const double* in, double* out, int B, int chunk) {
+    const int V = g.V, D = g.D;
+    const size_t npx = (size_t)g.H * g.W;
+    #pragma omp parallel for schedule(dynamic, 1) collapse(2)
+    for (int b = 0; b < B; b++) {
+        for (int v = 0; v < V; v++) {
+            const View& w = views[v];
+            double* img = out + ((size_t)b * V + v) * npx;
+            std::fill(img, img + npx, 0.0);
+            std::vector<double> gb(D);
+            for (int k = 0; k < D; k++) gb[k] = in[((size_t)b * V + v) * D + out_index(g, w, k)];
+            const int nlines = w.xmode ? g.H : g.W;
+            for (int l = 0; l < nlines; l++)
+                for (int c0 = 0; c0 < D; c0 += chunk)
+                    merge_to_pixels(g, w, c0, std::min(chunk, D - c0), l, &gb[c0], img, 1);
+        }
+    }
+}
+
 // RL taps h(n), n = -(D-1)..D-1 (fan_ed_kernel.cu:925-943).
 void rl_taps(int D, double delta, double* h) {
     const double PI = std::acos(-1.0);
@@ -493,6 +518,7 @@ void brute(const Geom& g, int op, const double* in, double* out, int B) {
     for (int v = 0; v < V; v++) bview(g, v, views[v]);
     bool to_sino = op == OP_PRJ || op == OP_BPRJ_T || op == OP_BPRJ_T_TRUE;
     if (to_sino) std::fill(out, out + (size_t)B * V * D, 0.0);
+    else if (op == OP_BPRJ_SV) std::fill(out, out + (size_t)B * V * H * W, 0.0);
     else std::fill(out, out + (size_t)B * H * W, 0.0);
     for (int v = 0; v < V; v++) {
         const BView& w = views[v];
@@ -546,6 +572,8 @@ void brute(const Geom& g, int op, const double* in, double* out, int B) {
                             out[((size_t)b * H + r) * W + c] += ov * fprj * in[((size_t)b * V + v) * D + j];
                         } else if (op == OP_BPRJ) {
                             out[((size_t)b * H + r) * W + c] += ov / sp * wt * in[((size_t)b * V + v) * D + j];
+                        } else if (op == OP_BPRJ_SV) {
+                            out[(((size_t)b * V + v) * H + r) * W + c] += ov / sp * wt * in[((size_t)b * V + v) * D + j];
                         } else {
                             double f = ov / sp * wt;
                             if (op == OP_BPRJ_T && g.scan == ST_EA) f *= g.dImg / cb[j];
@@ -602,6 +630,7 @@ int ctor_apply(int op, const double* opt, int nopt, const double* in, double* ou
     case OP_BPRJ_T_TRUE: run_to_sino(g, views, in, out, B, chunk, OP_BPRJ_T_TRUE); break;
     case OP_PRJ_T: run_to_image(g, views, in, out, B, chunk, OP_PRJ_T); break;
     case OP_BPRJ: run_to_image(g, views, in, out, B, chunk, OP_BPRJ); break;
+    case OP_BPRJ_SV: run_sv(g, views, in, out, B, chunk); break;
     case OP_FBP: {
         std::vector<double> q((size_t)B * g.V * g.D);
         ramp_rows(g, in, q.data(), B * g.V);
@@ -643,7 +672,7 @@ int ctor_brute(int op, const double* opt, int nopt, const double* in, double* ou
     Geom g;
     int rc = parse(opt, nopt, &g);
     if (rc) return rc;
-    if (op < 0 || op > 4) return E_ARG;
+    if (op < 0 || op > 6 || op == OP_FBP) return E_ARG;
     brute(g, op, in, out, B);
     return 0;
 }

@@ -7,7 +7,8 @@ Every function takes the options vector as the reference does (12 entries for
 fan beam, >= 9 for parallel beam, scan_type last; ctlib.cpp:18), converted to
 float64 from whatever dtype the caller used, and works on float64 numpy arrays
 shaped like the reference's tensors without the channel axis:
-image [B, H, W], sinogram [B, V, D].
+image [B, H, W], sinogram [B, V, D], per-view images (backprojection_sv)
+[B, V, H, W].
 """
 import ctypes
 import os
@@ -15,7 +16,7 @@ import subprocess
 
 import numpy as np
 
-OP_PRJ, OP_PRJ_T, OP_BPRJ, OP_BPRJ_T, OP_BPRJ_T_TRUE, OP_FBP = range(6)
+OP_PRJ, OP_PRJ_T, OP_BPRJ, OP_BPRJ_T, OP_BPRJ_T_TRUE, OP_FBP, OP_BPRJ_SV = range(7)
 OP_NAMES = {
     "projection": OP_PRJ,
     "projection_t": OP_PRJ_T,
@@ -23,6 +24,7 @@ OP_NAMES = {
     "backprojection_t": OP_BPRJ_T,
     "backprojection_t_true": OP_BPRJ_T_TRUE,
     "fbp": OP_FBP,
+    "backprojection_sv": OP_BPRJ_SV,
 }
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -103,6 +105,8 @@ def view(options, v):
 def _shape_in(op, g):
     if op in (OP_PRJ, OP_BPRJ_T, OP_BPRJ_T_TRUE):
         return (g["height"], g["width"]), (g["views"], g["dets"])
+    if op == OP_BPRJ_SV:
+        return (g["views"], g["dets"]), (g["views"], g["height"], g["width"])
     return (g["views"], g["dets"]), (g["height"], g["width"])
 
 

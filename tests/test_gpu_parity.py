@@ -102,10 +102,10 @@ def test_config0_parallel(dev, op):
     assert err <= F32_TOL
 
 
-@pytest.mark.parametrize("cfg", [1, 2])
+@pytest.mark.parametrize("cfg", [1, 2, 4])
 @pytest.mark.parametrize("op", ["projection", "projection_t", "backprojection", "backprojection_t", "fbp"])
 def test_full_fan_geometry(dev, cfg, op):
-    """Fan-ED 512^2 736x736 and fan-EA 512^2 1024x888 geometries (one batch slice)."""
+    """Fan-ED 512^2 736x736, fan-EA 512^2 1024x888 and fan-ED 1024^2 1440x1536 geometries (one batch slice)."""
     o = phantom.CONFIGS[cfg]["options"]
     g = O.parse(o)
     x_img = phantom.phantom_batch(1, g["height"], g["width"], config=cfg, scale=0.02)
@@ -208,3 +208,38 @@ def test_residual_step(dev):
     g_ref = ctlib_amd.projection_t(y_ref - m, ot)
     assert torch.equal(y, y_ref)
     assert rel_l2(grad.cpu().numpy(), g_ref.cpu().numpy()) <= 1e-6
+
+
+@pytest.mark.parametrize("G", [4, 8])
+@pytest.mark.parametrize("case", ["ed_small", "ea_odd", "para_ties", "config1"])
+def test_forward_launch_shapes(dev, case, G):
+    """Both line->bin launch shapes the autotuner may pick (4 or 8 views per block) give the
+    oracle's projection and residual step."""
+    from ctlib_amd import _C
+    if case == "config1":
+        o = phantom.CONFIGS[1]["options"]
+    elif case == "ed_small":
+        o = small_options(0, V=40, D=23, W=16, H=20)
+    elif case == "ea_odd":
+        o = small_options(1, V=33, D=31, W=13, H=11, Ang0=0.7)   # odd sizes, a partial view group
+    else:
+        o = phantom.para_options(8, 19, 10, 10, 0.1, 0.13, Ang0=0.0, dAng=math.pi / 4, binshift=0.0)
+    g = O.parse(o)
+    rng = np.random.default_rng(17)
+    x = rng.random((2, 1, g["height"], g["width"])).astype(np.float32)
+    mm = rng.random((2, 1, g["views"], g["dets"])).astype(np.float32)
+    ot = torch.from_numpy(o)
+    print(f"[case] forward views per block {G} {case}", flush=True)
+    try:
+        _C.set_plan_tuning(ot, dev.index or 0, G)
+        xt, mt = torch.from_numpy(x).to(dev), torch.from_numpy(mm).to(dev)
+        y = ctlib_amd.projection(xt, ot).cpu().numpy()
+        yr, gr = ctlib_amd.residual_step(xt, mt, ot)
+    finally:
+        _C.set_plan_tuning(ot, dev.index or 0, 0)
+    o64 = o.astype(np.float64)
+    y_ref = O.apply("projection", o64, x[:, 0].astype(np.float64))
+    g_ref = O.apply("projection_t", o64, y_ref - mm[:, 0].astype(np.float64))
+    assert rel_l2(y[:, 0], y_ref) <= F32_TOL
+    assert rel_l2(yr.cpu().numpy()[:, 0], y_ref) <= F32_TOL
+    assert rel_l2(gr.cpu().numpy()[:, 0], g_ref) <= F32_TOL

@@ -75,6 +75,33 @@ def test_merge_matches_brute_force(name, op):
 
 
 @pytest.mark.parametrize("name", list(GEOMS))
+def test_backprojection_sv_matches_brute_force(name):
+    o = GEOMS[name]
+    y = rand_in("backprojection", o, 2, 11)
+    a = O.apply("backprojection_sv", o, y)
+    b = O.brute("backprojection_sv", o, y)
+    assert a.shape == (2, y.shape[1]) + b.shape[2:]
+    assert rel(a, b) < 1e-13, (name, rel(a, b))
+
+
+@pytest.mark.parametrize("name", ["ed", "ea", "para", "ed_ties"])
+def test_backprojection_sv_is_per_view_backprojection(name):
+    """fan_ed_kernel.cu:945-1161 is bprj_fan_ed (:707-923) writing image[channel][view]:
+    image v is the backprojection of the sinogram with every other view zeroed,
+    and the views sum to the backprojection."""
+    o = GEOMS[name]
+    g = O.parse(o)
+    y = rand_in("backprojection", o, 2, 12)
+    sv = O.apply("backprojection_sv", o, y)
+    full = O.apply("backprojection", o, y)
+    np.testing.assert_allclose(sv.sum(axis=1), full, rtol=1e-12, atol=1e-13 * np.abs(full).max())
+    for v in (0, 3, g["views"] - 1):
+        ym = np.zeros_like(y)
+        ym[:, v] = y[:, v]
+        np.testing.assert_allclose(sv[:, v], O.apply("backprojection", o, ym), rtol=1e-13, atol=1e-300)
+
+
+@pytest.mark.parametrize("name", list(GEOMS))
 def test_adjoint_identities(name):
     o = GEOMS[name]
     g = O.parse(o)
