@@ -34,6 +34,29 @@ if ROOT not in sys.path:
 HBM_PEAK_GBS = 8000.0
 
 
+def calib_copy():
+    """One 256 MiB device-to-device copy: the reference dispatch for scaling HBM counters."""
+    import torch
+    src = torch.ones(64 << 20, dtype=torch.float32, device="cuda:0")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    del src, dst
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
+    summary (profiles/hbm_traffic.json), scaled by the copy-kernel calibration; None if absent."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    try:
+        with open(path) as f:
+            data = json.load(f)
+        k = data["kernels"][kernel]
+        return float(k["fetch_kib"]) * 1024 * data["scale"]["fetch"] + float(k["write_kib"]) * 1024 * data["scale"]["write"]
+    except (OSError, KeyError, ValueError, TypeError):
+        return None
+
+
 def workload_options(views=736):
     from ctlib_amd import phantom
     return phantom.fan_options(views, 736, 512, 512, 0.0066, 0.0072, 5.95, 4.906, 0, dAng=2 * math.pi / views)
@@ -163,7 +186,7 @@ def workload_sv(args):
         "config": {"workload": "fan-ED 512x512, 736 views x 736 dets, B=8: per-view backprojection",
                    "global_batch": B, "output_bytes": B * 736 * 512 * 512 * 4},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic("k_bprj_sv"),
                      "kernel": "k_bprj_sv", "kernel_ms": round(k_ms, 4), "algo_bytes_per_launch": k_bytes},
     }
 
@@ -251,6 +274,8 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU only; the projector shards over slices / views "
                                    "(see the design notes) but no multi-GPU path is measured", "n_gpus": args.gpus}))
         sys.exit(2)
+    if args.calib_copy:
+        calib_copy()
     if args.workload == "hetero":
         print(json.dumps(workload_hetero(args)), flush=True)
         return
@@ -267,12 +292,6 @@ def main():
     from ctlib_amd import _C, autograd, phantom
 
     dev = torch.device("cuda:0")
-    if args.calib_copy:
-        src = torch.ones(64 << 20, dtype=torch.float32, device=dev)
-        dst = torch.empty_like(src)
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        del src, dst
     B = args.batch
     opts = torch.from_numpy(workload_options())
     x_true = torch.from_numpy(phantom.phantom_batch(B, 512, 512, config=1, scale=0.02)).to(dev)
@@ -352,8 +371,9 @@ def main():
     per_launch_ms = st["ms"] / max(st["launches"], 1)
     per_launch_bytes = st["algo_bytes"] / max(st["launches"], 1)
     achieved = per_launch_bytes / (per_launch_ms * 1e-3) / 1e9
+    traffic = measured_traffic("k_fwd" if k == "fwd" else "k_adj") if B == 32 else None
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
             "kernel": "k_fwd (line->bin)" if k == "fwd" else "k_adj (bin->pixel)",
             "kernel_ms": round(per_launch_ms, 4), "algo_bytes_per_launch": per_launch_bytes}
     kernels = {c: {"launches": stats[c]["launches"],

@@ -55,6 +55,16 @@ inline int grid_for(int nwork) { return ((nwork + 7) / 8) * 8; }
 // Index of a cumulative table from a float coordinate, clamped to [0, hi].
 __device__ __forceinline__ int table_index(float q, float hi) { return (int)__builtin_amdgcn_fmed3f(q, 0.0f, hi); }
 
+// The same from a double coordinate: v_cvt_i32_f64 (truncation toward zero)
+// and one v_med3_i32 clamp to [0, hi] (written out because the compiler emits
+// a max/min pair for it).  Coordinates of valid geometries stay far inside the
+// int32 range.
+__device__ __forceinline__ int table_index_d(double q, int hi) {
+    int j = (int)q, r;
+    asm("v_med3_i32 %0, %1, 0, %2" : "=v"(r) : "v"(j), "v"(hi));
+    return r;
+}
+
 __device__ __forceinline__ double shfl_up_d(double v, int off) { return __shfl_up(v, off, 64); }
 __device__ __forceinline__ double shfl_down_d(double v, int off) { return __shfl_down(v, off, 64); }
 __device__ __forceinline__ double readlane_d(double v, int lane) {
@@ -243,7 +253,7 @@ __device__ __forceinline__ void fwd_block(const DevPlan& P, const T* __restrict_
     const T* base = src + (size_t)b * nlines * n;
     const int m_px = (n + NT - 1) / NT;
     const int p0 = tid * m_px;
-    const float hi = (float)(n + 1);
+    const int hi = n + 1;
 
     // coefficient word `cw` of view `cg` for the next line, held by thread tid < G*8
     const bool cthread = tid < G * LC_WORDS;
@@ -307,7 +317,7 @@ __device__ __forceinline__ void fwd_block(const DevPlan& P, const T* __restrict_
             for (int i = 0; i < NEM; i++) {
                 const double tau = __builtin_fma(A, e[v][i], Bc);
                 double c, sl;
-                lent_read(t + table_index((float)tau, hi), c, sl);
+                lent_read(t + table_index_d(tau, hi), c, sl);
                 S[v][i] = __builtin_fma(s, __builtin_fma(tau, sl, c), S[v][i]);
             }
         }
