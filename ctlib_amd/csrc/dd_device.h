@@ -474,10 +474,12 @@ __device__ __forceinline__ void adj_block(const DevPlan& P, const double2* __res
             const int l = tile * L + lq;
             if (l >= nlines) continue;    // wave-uniform
             const double a = cf[lq * LC_WORDS + LC_A0], s = cf[lq * LC_WORDS + LC_S];
+            // Every slot is evaluated, also past the last pixel edge (clamped table
+            // index, never stored): no branch between the independent evaluations,
+            // so their LDS reads overlap.
             if (!WEIGHTED) {
 #pragma unroll
-                for (int i = 0; i < NPE; i++)
-                    if (i < npe) acc[q][i] += eval_G<GEO>(t, a, s, lane + 64 * i, lk, ea_sc, ea_off, hi);
+                for (int i = 0; i < NPE; i++) acc[q][i] += eval_G<GEO>(t, a, s, lane + 64 * i, lk, ea_sc, ea_off, hi);
             } else {
                 const double w0 = cf[lq * LC_WORDS + LC_W0], w1 = cf[lq * LC_WORDS + LC_W1];
                 const double w2 = cf[lq * LC_WORDS + LC_W2];
@@ -490,27 +492,22 @@ __device__ __forceinline__ void adj_block(const DevPlan& P, const double2* __res
                 double gprev = 0.0;
 #pragma unroll
                 for (int i = 0; i < NPE; i++) {
-                    if (i < npe) {
-                        const double gi = eval_G<GEO>(t, a, s, lane + 64 * i, lk, ea_sc, ea_off, hi);
-                        if (i > 0) {
-                            double right = shfl_down_d(gprev, 1);
-                            const double r0 = readlane_d(gi, 0);
-                            if (lane == 63) right = r0;
-                            const int p = lane + 64 * (i - 1);
-                            if (p < n) acc[q][i - 1] += wgt(p) * (right - gprev);
-                        }
-                        gprev = gi;
+                    const double gi = eval_G<GEO>(t, a, s, lane + 64 * i, lk, ea_sc, ea_off, hi);
+                    if (i > 0) {
+                        double right = shfl_down_d(gprev, 1);
+                        const double r0 = readlane_d(gi, 0);
+                        if (lane == 63) right = r0;
+                        const int p = lane + 64 * (i - 1);
+                        acc[q][i - 1] += p < n ? wgt(p) * (right - gprev) : 0.0;
                     }
+                    gprev = gi;
                 }
-                // pixels of the last edge group (their right edges are in the same group)
+                // pixels of the last slot group (their right edges are in the same group;
+                // lane 63's pixel is past the line since 64 * NPE > n)
                 {
                     const double right = shfl_down_d(gprev, 1);
-                    const int p = lane + 64 * (npe - 1);
-                    if (p < n) {
-#pragma unroll
-                        for (int i = 0; i < NPE; i++)
-                            if (i == npe - 1) acc[q][i] += wgt(p) * (right - gprev);
-                    }
+                    const int p = lane + 64 * (NPE - 1);
+                    acc[q][NPE - 1] += p < n ? wgt(p) * (right - gprev) : 0.0;
                 }
             }
         }
