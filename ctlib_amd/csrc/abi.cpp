@@ -381,6 +381,9 @@ int ctlib_plan_create(const ctlib_geom* geom, int device, ctlib_plan** out) {
     P.max_lines = H.max_lines;
     P.scan_type = g.scan_type;
     P.dAng = g.dAng;
+    for (int k = 0; k < 6; k++) P.ea_c[k] = H.ea_c[k];
+    P.ea_T = H.ea_T;
+    P.ea_poly = H.ea_poly;
     *out = p;
     return CTLIB_OK;
 }
@@ -443,6 +446,7 @@ int ctlib_plan_export(const ctlib_geom* geom, int table, void* dst, size_t* byte
     if (rc) return fail(rc, why);
     const void* src = nullptr;
     size_t n = 0;
+    float ea[8];
     switch (table) {
     case 0: src = H.vp.data(); n = H.vp.size() * sizeof(ViewParams); break;
     case 1: src = H.edges.data(); n = H.edges.size() * sizeof(double); break;
@@ -453,6 +457,11 @@ int ctlib_plan_export(const ctlib_geom* geom, int table, void* dst, size_t* byte
     case 6: src = H.yviews.data(); n = H.yviews.size() * sizeof(int); break;
     case 7: src = H.ramp_taps.data(); n = H.ramp_taps.size() * sizeof(double); break;
     case 8: src = H.ramp_preweight.data(); n = H.ramp_preweight.size() * sizeof(double); break;
+    case 9:
+        ea[0] = H.ea_c[0]; ea[1] = H.ea_c[1]; ea[2] = H.ea_c[2]; ea[3] = H.ea_c[3]; ea[4] = H.ea_c[4];
+        ea[5] = H.ea_c[5]; ea[6] = H.ea_T; ea[7] = (float)H.ea_poly;
+        src = ea; n = sizeof(ea);
+        break;
     default: return fail(CTLIB_ERR_INVALID_ARG, "unknown table");
     }
     if (dst) {

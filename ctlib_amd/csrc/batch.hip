@@ -46,7 +46,7 @@ struct SliceDev {
     double2* tabs;
     void* yt;
     int fcls;   // forward variant: edge-slot class
-    int acls;   // adjoint variant: NPE class * 3 + GEO
+    int acls;   // adjoint variant: NPE class * 4 + GEO (GEO_EAP for fan-EA with the polynomial atan)
     int ngx, ngy, ntx, nty, nband;
     int pad[3];
 };
@@ -75,9 +75,11 @@ __device__ __forceinline__ void run_adj(const SliceDev& sd, int mode, int tile, 
     case 0: adj_block<T, 8, 5, false, GEO_ED>(sd.P, sd.tabs, ox, oy, 0, mode, tile, 1.0, smem); break;
     case 1: adj_block<T, 8, 5, false, GEO_EA>(sd.P, sd.tabs, ox, oy, 0, mode, tile, 1.0, smem); break;
     case 2: adj_block<T, 8, 5, false, GEO_PARA>(sd.P, sd.tabs, ox, oy, 0, mode, tile, 1.0, smem); break;
-    case 3: adj_block<T, 4, 9, false, GEO_ED>(sd.P, sd.tabs, ox, oy, 0, mode, tile, 1.0, smem); break;
-    case 4: adj_block<T, 4, 9, false, GEO_EA>(sd.P, sd.tabs, ox, oy, 0, mode, tile, 1.0, smem); break;
-    default: adj_block<T, 4, 9, false, GEO_PARA>(sd.P, sd.tabs, ox, oy, 0, mode, tile, 1.0, smem); break;
+    case 3: adj_block<T, 8, 5, false, GEO_EAP>(sd.P, sd.tabs, ox, oy, 0, mode, tile, 1.0, smem); break;
+    case 4: adj_block<T, 4, 9, false, GEO_ED>(sd.P, sd.tabs, ox, oy, 0, mode, tile, 1.0, smem); break;
+    case 5: adj_block<T, 4, 9, false, GEO_EA>(sd.P, sd.tabs, ox, oy, 0, mode, tile, 1.0, smem); break;
+    case 6: adj_block<T, 4, 9, false, GEO_PARA>(sd.P, sd.tabs, ox, oy, 0, mode, tile, 1.0, smem); break;
+    default: adj_block<T, 4, 9, false, GEO_EAP>(sd.P, sd.tabs, ox, oy, 0, mode, tile, 1.0, smem); break;
     }
 }
 
@@ -228,7 +230,7 @@ void build_items(ctlib_batch* b) {
             ph[IT_F].push_back({(double)P.H * (P.D + 1) * BG, Item{IT_F, s, MODE_X, g}});
         for (int g = 0; g < sd.ngy; g++)
             ph[IT_F].push_back({(double)P.W * (P.D + 1) * BG, Item{IT_F, s, MODE_Y, g}});
-        const int L = (sd.acls < 3) ? 32 : 16;
+        const int L = (sd.acls < 4) ? 32 : 16;
         for (int t = 0; t < sd.ntx; t++) ph[IT_A].push_back({(double)L * (P.W + 1) * P.nx, Item{IT_A, s, MODE_X, t}});
         for (int t = 0; t < sd.nty; t++) ph[IT_A].push_back({(double)L * (P.H + 1) * P.ny, Item{IT_A, s, MODE_Y, t}});
         if (P.ny)
@@ -288,7 +290,7 @@ int ctlib_batch_create(const ctlib_geom* geoms, int nslices, int device, int dty
         sd.fcls = ne <= 12 ? 0 : 1;
         const int npe = (nmax + 1 + 63) / 64;
         const int npecls = npe <= 5 ? 0 : 1;
-        sd.acls = npecls * 3 + P.scan_type;
+        sd.acls = npecls * 4 + (P.scan_type == GEO_EA && P.ea_poly ? GEO_EAP : P.scan_type);
         const int L = npecls == 0 ? 32 : 16;
         sd.ngx = (P.nx + BG - 1) / BG;
         sd.ngy = (P.ny + BG - 1) / BG;

@@ -39,7 +39,7 @@ namespace dd {
 constexpr int NT = 256;      // threads per block (4 wave64)
 constexpr int LC_WORDS = 8;  // doubles per LineCoef: a, s, A, B, w0, w1, w2, inv_s
 enum { LC_A0 = 0, LC_S = 1, LC_A = 2, LC_B = 3, LC_W0 = 4, LC_W1 = 5, LC_W2 = 6, LC_INVS = 7 };
-enum { GEO_ED = 0, GEO_EA = 1, GEO_PARA = 2 };
+enum { GEO_ED = 0, GEO_EA = 1, GEO_PARA = 2, GEO_EAP = 3 };  // EAP: fan-EA, polynomial atan
 
 // Blocks are dispatched round-robin over the 8 XCDs (block id % 8).  Give each
 // XCD a contiguous range of work items so that blocks sharing an image or a
@@ -374,19 +374,49 @@ __device__ __forceinline__ void fwd_block(const DevPlan& P, const T* __restrict_
 // Bin -> pixel (projection_t / backprojection)
 // ---------------------------------------------------------------------------
 
+// Axis coordinate -> bin lookup of one view (uniform values, held in SGPRs).
+struct Lookup {
+    float lk[4];      // Moebius (ED, EA) or affine (parallel) coefficients
+    float ea_sc, ea_off;
+    float c[6], T;    // fan-EA odd polynomial for atan on |t| <= T (GEO_EAP)
+};
+
+__device__ __forceinline__ Lookup view_lookup(const DevPlan& P, const ViewParams& vp) {
+    Lookup L;
+#pragma unroll
+    for (int k = 0; k < 4; k++) L.lk[k] = vp.lk[k];
+    L.ea_sc = vp.ea_sc;
+    L.ea_off = vp.ea_off;
+#pragma unroll
+    for (int k = 0; k < 6; k++) L.c[k] = P.ea_c[k];
+    L.T = P.ea_T;
+    return L;
+}
+
+// G(pi_p) of pixel edge p of a line with edges pi_p = a + s*p.  GEO_EA takes the
+// arctangent with atanf; GEO_EAP with the plan's odd polynomial (t clamped to
+// [-T, T], beyond which the index clamps to an end entry anyway).
 template <int GEO>
-__device__ __forceinline__ double eval_G(const double2* __restrict__ t, double a, double s, int p, const float (&lk)[4],
-                                         float ea_sc, float ea_off, float hi) {
+__device__ __forceinline__ double eval_G(const double2* __restrict__ t, double a, double s, int p, const Lookup& L,
+                                         float hi) {
     const double pi = __builtin_fma(s, (double)p, a);
     const float pf = (float)pi;
     float q;
     if (GEO == GEO_PARA) {
-        q = __builtin_fmaf(lk[0], pf, lk[1]);
+        q = __builtin_fmaf(L.lk[0], pf, L.lk[1]);
     } else {
-        const float num = __builtin_fmaf(lk[0], pf, lk[1]);
-        const float den = __builtin_fmaf(lk[2], pf, lk[3]);
+        const float num = __builtin_fmaf(L.lk[0], pf, L.lk[1]);
+        const float den = __builtin_fmaf(L.lk[2], pf, L.lk[3]);
         q = num * __builtin_amdgcn_rcpf(den);
-        if (GEO == GEO_EA) q = __builtin_fmaf(atanf(q), ea_sc, ea_off);
+        if (GEO == GEO_EA) q = __builtin_fmaf(atanf(q), L.ea_sc, L.ea_off);
+        if (GEO == GEO_EAP) {
+            const float tc = __builtin_amdgcn_fmed3f(q, -L.T, L.T);
+            const float s2 = tc * tc;
+            float z = L.c[5];
+#pragma unroll
+            for (int k = 4; k >= 0; k--) z = __builtin_fmaf(z, s2, L.c[k]);
+            q = __builtin_fmaf(tc * z, L.ea_sc, L.ea_off);
+        }
     }
     const double2 c = t[table_index(q, hi)];
     return __builtin_fma(pi, c.y, c.x);
@@ -466,8 +496,7 @@ __device__ __forceinline__ void adj_block(const DevPlan& P, const double2* __res
         const double2* t = stab + cur * tabn;
         const double* cf = scf + cur * L * LC_WORDS;
         const ViewParams& vp = P.vp[v];
-        const float lk[4] = {vp.lk[0], vp.lk[1], vp.lk[2], vp.lk[3]};
-        const float ea_sc = vp.ea_sc, ea_off = vp.ea_off;
+        const Lookup lu = view_lookup(P, vp);
 #pragma unroll
         for (int q = 0; q < LPW; q++) {
             const int lq = q * 4 + wave;  // line within the tile
@@ -479,7 +508,7 @@ __device__ __forceinline__ void adj_block(const DevPlan& P, const double2* __res
             // so their LDS reads overlap.
             if (!WEIGHTED) {
 #pragma unroll
-                for (int i = 0; i < NPE; i++) acc[q][i] += eval_G<GEO>(t, a, s, lane + 64 * i, lk, ea_sc, ea_off, hi);
+                for (int i = 0; i < NPE; i++) acc[q][i] += eval_G<GEO>(t, a, s, lane + 64 * i, lu, hi);
             } else {
                 const double w0 = cf[lq * LC_WORDS + LC_W0], w1 = cf[lq * LC_WORDS + LC_W1];
                 const double w2 = cf[lq * LC_WORDS + LC_W2];
@@ -492,7 +521,7 @@ __device__ __forceinline__ void adj_block(const DevPlan& P, const double2* __res
                 double gprev = 0.0;
 #pragma unroll
                 for (int i = 0; i < NPE; i++) {
-                    const double gi = eval_G<GEO>(t, a, s, lane + 64 * i, lk, ea_sc, ea_off, hi);
+                    const double gi = eval_G<GEO>(t, a, s, lane + 64 * i, lu, hi);
                     if (i > 0) {
                         double right = shfl_down_d(gprev, 1);
                         const double r0 = readlane_d(gi, 0);

@@ -130,8 +130,7 @@ __global__ __launch_bounds__(256) void k_bprj_sv(DevPlan P, const double2* __res
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const ViewParams& vp = P.vp[v];
-    const float lk[4] = {vp.lk[0], vp.lk[1], vp.lk[2], vp.lk[3]};
-    const float ea_sc = vp.ea_sc, ea_off = vp.ea_off;
+    const Lookup lu = view_lookup(P, vp);
     const float hi = (float)(D + 1);
     const LineCoef* lc = P.lines + (size_t)v * P.max_lines;
     T* img = out + (size_t)item * H * W;
@@ -147,7 +146,7 @@ __global__ __launch_bounds__(256) void k_bprj_sv(DevPlan P, const double2* __res
             T* dst = img + (size_t)r * W;
             double gprev = 0.0;
             for (int i = 0; i < npe; i++) {
-                const double gi = eval_G<GEO>(t, c.a, c.s, lane + 64 * i, lk, ea_sc, ea_off, hi);
+                const double gi = eval_G<GEO>(t, c.a, c.s, lane + 64 * i, lu, hi);
                 if (i > 0) {
                     double right = shfl_down_d(gprev, 1);
                     const double r0 = readlane_d(gi, 0);
@@ -166,10 +165,10 @@ __global__ __launch_bounds__(256) void k_bprj_sv(DevPlan P, const double2* __res
         for (int c0 = wave * 64; c0 < W; c0 += NT) {
             const int col = c0 + lane;
             const LineCoef c = lc[col < W ? col : W - 1];
-            double gprev = eval_G<GEO>(t, c.a, c.s, 0, lk, ea_sc, ea_off, hi);
+            double gprev = eval_G<GEO>(t, c.a, c.s, 0, lu, hi);
             T* dst = img + (size_t)(H - 1) * W + col;
             for (int i = 0; i < H; i++) {
-                const double gi = eval_G<GEO>(t, c.a, c.s, i + 1, lk, ea_sc, ea_off, hi);
+                const double gi = eval_G<GEO>(t, c.a, c.s, i + 1, lu, hi);
                 if (col < W) dst[-(ptrdiff_t)i * W] = (T)(wgt(c, i) * (gi - gprev));
                 gprev = gi;
             }
@@ -334,7 +333,9 @@ hipError_t adj_geo(const DevPlan& P, const double2* tabs, void* outX, void* outY
                    hipStream_t s) {
     switch (P.scan_type) {
     case GEO_ED: return adj_npe<T, WTD, GEO_ED>(P, tabs, outX, outYT, B, wscale, s);
-    case GEO_EA: return adj_npe<T, WTD, GEO_EA>(P, tabs, outX, outYT, B, wscale, s);
+    case GEO_EA:
+        return P.ea_poly ? adj_npe<T, WTD, GEO_EAP>(P, tabs, outX, outYT, B, wscale, s)
+                         : adj_npe<T, WTD, GEO_EA>(P, tabs, outX, outYT, B, wscale, s);
     default: return adj_npe<T, WTD, GEO_PARA>(P, tabs, outX, outYT, B, wscale, s);
     }
 }
@@ -367,7 +368,9 @@ template <typename T>
 hipError_t sv_geo(const DevPlan& P, const double2* tabs, void* out, int B, double wscale, hipStream_t s) {
     switch (P.scan_type) {
     case GEO_ED: return sv_launch_t<T, GEO_ED>(P, tabs, out, B, wscale, s);
-    case GEO_EA: return sv_launch_t<T, GEO_EA>(P, tabs, out, B, wscale, s);
+    case GEO_EA:
+        return P.ea_poly ? sv_launch_t<T, GEO_EAP>(P, tabs, out, B, wscale, s)
+                         : sv_launch_t<T, GEO_EA>(P, tabs, out, B, wscale, s);
     default: return sv_launch_t<T, GEO_PARA>(P, tabs, out, B, wscale, s);
     }
 }

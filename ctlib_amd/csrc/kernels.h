@@ -24,6 +24,9 @@ struct DevPlan {
     int V, D, W, H, max_lines;
     int scan_type;
     double dAng;
+    float ea_c[6];            // fan-EA polynomial atan (see HostPlan)
+    float ea_T;
+    int ea_poly;
 };
 
 enum { DT_F32 = 0, DT_F64 = 1 };

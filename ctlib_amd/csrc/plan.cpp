@@ -21,6 +21,52 @@ std::string fmt(const char* f, double a = 0, double b = 0) {
     return buf;
 }
 
+// Odd polynomial for the fan-EA axis -> bin lookup: atan(t) on |t| <= T with
+// T = tan(largest |fan angle| of a bin edge + one bin), from the Chebyshev
+// expansion of atan(T x) on [-1, 1] (odd terms T_1 .. T_11, coefficients by the
+// discrete cosine sum over 64 Chebyshev nodes), converted to powers of t.  A
+// ray outside the fan has |t| > T and is clamped to T, which maps past the last
+// bin edge as the exact arctangent would.  Kept only if the fp32-rounded
+// coefficients stay within 5e-8 rad of atan on [0, T] (fans up to about 60
+// degrees each side).
+void fit_atan(const Geom& g, HostPlan* P) {
+    const double PI = std::acos(-1.0);
+    const double hD = g.dets / 2.0;
+    const double gmax = std::fmax(std::fabs(-hD * g.dDet + g.binshift), std::fabs(hD * g.dDet + g.binshift)) + g.dDet;
+    P->ea_poly = 0;
+    if (!(gmax < 1.05)) return;
+    const double T = std::tan(gmax);
+    constexpr int N = 64, NT = 12;  // nodes; Chebyshev terms 0..11
+    double a[NT] = {};
+    for (int j = 1; j < NT; j += 2) {
+        double s = 0.0;
+        for (int k = 0; k < N; k++) {
+            const double th = PI * (k + 0.5) / N;
+            s += std::atan(T * std::cos(th)) * std::cos(j * th);
+        }
+        a[j] = 2.0 * s / N;
+    }
+    // power coefficients of sum_j a_j T_j(x) through T_{j+1} = 2x T_j - T_{j-1}
+    double Tm[NT][NT] = {}, b[NT] = {};
+    Tm[0][0] = 1.0;
+    Tm[1][1] = 1.0;
+    for (int j = 2; j < NT; j++)
+        for (int i = 0; i < NT; i++) Tm[j][i] = (i > 0 ? 2.0 * Tm[j - 1][i - 1] : 0.0) - Tm[j - 2][i];
+    for (int j = 1; j < NT; j += 2)
+        for (int i = 0; i < NT; i++) b[i] += a[j] * Tm[j][i];
+    // atan(t) ~ sum_k c_k t^(2k+1), c_k = b_(2k+1) / T^(2k+1)
+    for (int k = 0; k < 6; k++) P->ea_c[k] = (float)(b[2 * k + 1] / std::pow(T, 2 * k + 1));
+    P->ea_T = (float)T;
+    double err = 0.0;
+    for (int i = 0; i <= 4000; i++) {
+        const double t = T * i / 4000.0, s = t * t;
+        double p = P->ea_c[5];
+        for (int k = 4; k >= 0; k--) p = p * s + P->ea_c[k];
+        err = std::fmax(err, std::fabs(t * p - std::atan(t)));
+    }
+    P->ea_poly = err <= 5e-8 ? 1 : 0;
+}
+
 }  // namespace
 
 int parse_options(const double* o, int n, Geom* g, std::string* why) {
@@ -251,6 +297,8 @@ int build_plan(const Geom& g, HostPlan* P, std::string* why) {
             lc[l] = c;
         }
     }
+
+    if (g.scan_type == SCAN_FAN_EA) fit_atan(g, P);
 
     // fbp: RL taps (fan_ed_kernel.cu:925-943) and pre-weight (fan_ed_kernel.cu:1349-1355;
     // fan_ea_kernel.cu:1075-1080; para_kernel.cu:823), both in output bin order.

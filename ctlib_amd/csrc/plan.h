@@ -57,6 +57,11 @@ struct HostPlan {
     std::vector<int> xviews, yviews;     // view indices per sweep axis, ascending
     std::vector<double> ramp_taps;       // [2D-1] RL taps h(-(D-1)..D-1)
     std::vector<double> ramp_preweight;  // [D] fbp pre-weight (output bin order)
+    // fan-EA lookup: atan(t) ~ t * (c0 + c1 t^2 + ... + c5 t^10) for |t| <= ea_T (t clamped there),
+    // used when ea_poly (the fit is within 5e-8 rad of atan on [0, ea_T]); else the kernels call atanf
+    float ea_c[6] = {0, 0, 0, 0, 0, 0};
+    float ea_T = 0.0f;
+    int ea_poly = 0;
 };
 
 // Parses the reference options vector (ctlib.cpp:18; fan_ed_kernel.cu:1165-1175;

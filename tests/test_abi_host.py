@@ -127,3 +127,24 @@ def test_kernel_model_parallel_rows_at_full_config0():
     got = M.apply(P, "projection", x)
     ref = O.apply("projection", o.astype(np.float64), x)
     assert np.linalg.norm(got - ref) / np.linalg.norm(ref) < 1e-13
+
+
+@pytest.mark.parametrize("dDet, expect_poly", [(0.03, True), (0.0011, True), (0.0005, True), (0.0016, False)])
+def test_fan_ea_atan_polynomial(dDet, expect_poly):
+    """The fan-EA lookup replaces atanf by an odd polynomial on |t| <= T: T must cover every bin edge's
+    fan angle (plus one bin), and the fp32 coefficients must stay within 5e-8 rad of atan there;
+    wider fans keep atanf."""
+    D = 888 if dDet < 0.01 else 17
+    o = phantom.fan_options(64, D, 32, 32, 0.05 if dDet >= 0.01 else 0.0066, dDet, 5.95, 4.906, 1, binshift=0.0003)
+    ea = abi.plan_export(o, "ea_atan")
+    c, T, use = ea[:6].astype(np.float64), float(ea[6]), bool(ea[7])
+    assert use == expect_poly
+    if not use:
+        return
+    gmax = max(abs(-D / 2 * dDet + 0.0003), abs(D / 2 * dDet + 0.0003))
+    assert T >= np.tan(gmax)
+    t = np.linspace(0.0, T, 20001)
+    p = t * np.polyval(c[::-1], t * t)
+    assert np.max(np.abs(p - np.arctan(t))) <= 5e-8
+    # monotone on [0, T], so clamped rays beyond T stay beyond the last edge
+    assert np.all(np.diff(p) > 0)
