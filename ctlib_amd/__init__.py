@@ -29,7 +29,9 @@ except ImportError as exc:  # pragma: no cover - exercised only on broken instal
         "ctlib_amd: the HIP extension (ctlib_amd/_C.so + libctlib_amd.so) is missing or failed to load; "
         "build it with `python -m ctlib_amd.build`") from exc
 
-__version__ = "0.2.0"
+from . import ops  # noqa: E402,F401  (registers the torch.ops.ctlib_amd operators)
+
+__version__ = "0.3.0"
 
 
 def projection(image, options):
