@@ -110,11 +110,14 @@ __global__ __launch_bounds__(256) void k_combine(T* __restrict__ out, const T* _
 // backprojection_sv: one view's term of the weighted backprojection, written to
 // its own image, out[b][v][r][c] (fan_ed_kernel.cu:945-1161).  One block per
 // (b, view); the view's bin table is staged in LDS.  X-axis views: a wave owns
-// an image row and its lanes the pixel edges of 64 consecutive columns (the
-// right edge comes from the neighbouring lane), so stores are row-contiguous.
-// Y-axis views: each lane owns a column and walks its pixels bottom to top, so
+// an image row and its lanes the pixel edges of 64 consecutive columns (NPE
+// groups, all evaluated before the stores; the right edge comes from the
+// neighbouring lane), so stores are row-contiguous.  Y-axis views: each lane
+// owns a column and walks its pixels bottom to top, SV_UNROLL at a time, so
 // the 64 lanes of a wave store 64 adjacent columns of one row.
-template <typename T, int GEO>
+constexpr int SV_UNROLL = 8;
+
+template <typename T, int GEO, int NPE>
 __global__ __launch_bounds__(256) void k_bprj_sv(DevPlan P, const double2* __restrict__ tabs, T* __restrict__ out,
                                                  int nitems, double wscale) {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -140,36 +143,46 @@ __global__ __launch_bounds__(256) void k_bprj_sv(DevPlan P, const double2* __res
         return c.inv_s * wscale * (double)__builtin_amdgcn_rcpf((float)d);
     };
     if (vp.xmode) {
-        const int npe = (W + 1 + 63) / 64;  // edge groups of a row
         for (int r = wave; r < H; r += NT / 64) {
             const LineCoef c = lc[r];
             T* dst = img + (size_t)r * W;
-            double gprev = 0.0;
-            for (int i = 0; i < npe; i++) {
-                const double gi = eval_G<GEO>(t, c.a, c.s, lane + 64 * i, lu, hi);
-                if (i > 0) {
-                    double right = shfl_down_d(gprev, 1);
-                    const double r0 = readlane_d(gi, 0);
+            double g[NPE];
+#pragma unroll
+            for (int i = 0; i < NPE; i++) g[i] = eval_G<GEO>(t, c.a, c.s, lane + 64 * i, lu, hi);
+#pragma unroll
+            for (int i = 0; i < NPE; i++) {
+                double right = shfl_down_d(g[i], 1);
+                if (i + 1 < NPE) {
+                    const double r0 = readlane_d(g[i + 1], 0);
                     if (lane == 63) right = r0;
-                    const int p = lane + 64 * (i - 1);
-                    if (p < W) dst[p] = (T)(wgt(c, p) * (right - gprev));
                 }
-                gprev = gi;
+                // lane 63 of the last group is past the row: 64 * NPE > W
+                const int p = lane + 64 * i;
+                if (p < W) dst[p] = (T)(wgt(c, p) * (right - g[i]));
             }
-            // pixels of the last group: their right edges are in the same group
-            const double right = shfl_down_d(gprev, 1);
-            const int p = lane + 64 * (npe - 1);
-            if (p < W) dst[p] = (T)(wgt(c, p) * (right - gprev));
         }
     } else {
         for (int c0 = wave * 64; c0 < W; c0 += NT) {
             const int col = c0 + lane;
-            const LineCoef c = lc[col < W ? col : W - 1];
+            const bool ok = col < W;
+            const LineCoef c = lc[ok ? col : W - 1];
             double gprev = eval_G<GEO>(t, c.a, c.s, 0, lu, hi);
             T* dst = img + (size_t)(H - 1) * W + col;
-            for (int i = 0; i < H; i++) {
+            int i = 0;
+            for (; i + SV_UNROLL <= H; i += SV_UNROLL) {
+                double g[SV_UNROLL];
+#pragma unroll
+                for (int u = 0; u < SV_UNROLL; u++) g[u] = eval_G<GEO>(t, c.a, c.s, i + u + 1, lu, hi);
+#pragma unroll
+                for (int u = 0; u < SV_UNROLL; u++) {
+                    const double left = u ? g[u - 1] : gprev;
+                    if (ok) dst[-(ptrdiff_t)(i + u) * W] = (T)(wgt(c, i + u) * (g[u] - left));
+                }
+                gprev = g[SV_UNROLL - 1];
+            }
+            for (; i < H; i++) {
                 const double gi = eval_G<GEO>(t, c.a, c.s, i + 1, lu, hi);
-                if (col < W) dst[-(ptrdiff_t)i * W] = (T)(wgt(c, i) * (gi - gprev));
+                if (ok) dst[-(ptrdiff_t)i * W] = (T)(wgt(c, i) * (gi - gprev));
                 gprev = gi;
             }
         }
@@ -355,13 +368,24 @@ hipError_t launch_adj(const DevPlan& P, int dtype, const double2* tabs, void* ou
 
 namespace {
 
-template <typename T, int GEO>
-hipError_t sv_launch_t(const DevPlan& P, const double2* tabs, void* out, int B, double wscale, hipStream_t s) {
+template <typename T, int GEO, int NPE>
+hipError_t sv_launch_npe(const DevPlan& P, const double2* tabs, void* out, int B, double wscale, hipStream_t s) {
     const int nitems = B * P.V;
     const size_t lds = (size_t)(P.D + 2) * sizeof(double2);
-    hipLaunchKernelGGL((k_bprj_sv<T, GEO>), dim3(grid_for(nitems)), dim3(NT), lds, s, P, tabs, (T*)out, nitems,
+    hipLaunchKernelGGL((k_bprj_sv<T, GEO, NPE>), dim3(grid_for(nitems)), dim3(NT), lds, s, P, tabs, (T*)out, nitems,
                        wscale);
     return hipGetLastError();
+}
+
+// Edge groups of an X-axis row: 64 * NPE >= W + 1.
+template <typename T, int GEO>
+hipError_t sv_launch_t(const DevPlan& P, const double2* tabs, void* out, int B, double wscale, hipStream_t s) {
+    const int npe = (P.W + 1 + 63) / 64;
+    if (npe <= 5) return sv_launch_npe<T, GEO, 5>(P, tabs, out, B, wscale, s);
+    if (npe <= 9) return sv_launch_npe<T, GEO, 9>(P, tabs, out, B, wscale, s);
+    if (npe <= 17) return sv_launch_npe<T, GEO, 17>(P, tabs, out, B, wscale, s);
+    if (npe <= 33) return sv_launch_npe<T, GEO, 33>(P, tabs, out, B, wscale, s);
+    return hipErrorInvalidValue;
 }
 
 template <typename T>
