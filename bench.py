@@ -145,6 +145,68 @@ def workload_hetero(args):
     }
 
 
+def workload_ops(args):
+    """Every public operator on the reference's fan-ED configuration (512^2, 736 x 736, B=8) and on
+    fan-EA (512^2, 1024 x 888, B=16): call time, sweep-model rate of its DD kernel (library events),
+    and the FP64 oracle on the host for the same operator on a bounded sample of the views."""
+    import torch
+
+    import ctlib_amd
+    from ctlib_amd import _C, phantom
+    from oracle import oracle as O
+
+    dev = torch.device("cuda:0")
+    names = ("projection", "projection_t", "backprojection", "backprojection_t", "fbp")
+    img_in = ("projection", "backprojection_t")
+    out = {"metric": "per-operator calls/s and sweep-model %HBM", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "dtype": "f32", "data": "synthetic", "configs": {}}
+    for cfg, B in ((1, 8), (2, 16)):
+        o = phantom.CONFIGS[cfg]["options"]
+        ot = torch.from_numpy(o)
+        g = ctlib_amd.parse_options(ot)
+        x = torch.from_numpy(phantom.phantom_batch(B, g["height"], g["width"], config=cfg, scale=0.02)).to(dev)
+        y = ctlib_amd.projection(x, ot)
+        sweep = 4.0 * B * g["views"] * (g["height"] * g["width"] + g["dets"])
+        rows = {}
+        for name in names:
+            f, inp = getattr(ctlib_amd, name), (x if name in img_in else y)
+            for _ in range(args.warmup):
+                f(inp, ot)
+            torch.cuda.synchronize()
+            _C.timing_collect()
+            _C.timing_enable(True)
+            t0 = time.perf_counter()
+            for _ in range(args.steps):
+                f(inp, ot)
+            torch.cuda.synchronize()
+            ms = (time.perf_counter() - t0) * 1e3 / args.steps
+            _C.timing_enable(False)
+            st = _C.timing_collect()
+            k = "fwd" if name in img_in else "adj"
+            k_ms = st[k]["ms"] / max(st[k]["launches"], 1)
+            row = {"ms_per_call": round(ms, 4), "calls_per_s": round(1e3 / ms, 2),
+                   "dd_kernel_ms": round(k_ms, 4), "dd_kernel_frac_8TBs": round(sweep / (k_ms * 1e-3) / 8e12, 4)}
+            if name == "fbp":
+                r_ms = st["ramp"]["ms"] / max(st["ramp"]["launches"], 1)
+                row["ramp_ms"] = round(r_ms, 4)
+                row["ramp_frac_fp64_mfma"] = round(st["ramp"]["flops"] / max(st["ramp"]["launches"], 1)
+                                                   / (r_ms * 1e-3) / 78.6e12, 4)
+            if not args.no_cpu:
+                # FP64 oracle, same operator, one slice over a view subset, scaled to the call
+                vs = max(8, g["views"] // 16)
+                o_sub = o.astype(np.float64).copy()
+                o_sub[0] = vs
+                xin = (x[:1, 0] if name in img_in else y[:1, 0, :vs]).double().cpu().numpy()
+                t0 = time.perf_counter()
+                O.apply(name, o_sub, xin)
+                dt = time.perf_counter() - t0
+                row["cpu_oracle_calls_per_s"] = round(1.0 / (dt * (g["views"] / vs) * B), 5)
+                row["cpu_sample"] = f"1 slice x {vs} views, {O.max_threads()} threads, scaled"
+            rows[name] = row
+        out["configs"][f"{'fan-ED' if cfg == 1 else 'fan-EA'} {g['height']}^2 {g['views']}x{g['dets']} B={B}"] = rows
+    return out
+
+
 def workload_sv(args):
     """backprojection_sv (per-view backprojection) on fan-ED 512^2, 736 views x 736 bins, B=8.
 
@@ -266,9 +328,10 @@ def main():
     ap.add_argument("--paths", default="plain,autograd,graph")
     ap.add_argument("--calib-copy", action="store_true",
                     help="run one 256 MiB device copy first (reference kernel for HBM counter scaling)")
-    ap.add_argument("--workload", default="metric", choices=("metric", "hetero", "unroll", "sv"),
+    ap.add_argument("--workload", default="metric", choices=("metric", "hetero", "unroll", "sv", "ops"),
                     help="metric: the headline fan-ED B=32 step (default); hetero: 64-slice batch; "
-                         "unroll: 50-iteration checkpointed reconstruction")
+                         "unroll: 50-iteration checkpointed reconstruction; sv: backprojection_sv; "
+                         "ops: every public operator")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; the projector shards over slices / views "
@@ -284,6 +347,9 @@ def main():
         return
     if args.workload == "sv":
         print(json.dumps(workload_sv(args)), flush=True)
+        return
+    if args.workload == "ops":
+        print(json.dumps(workload_ops(args)), flush=True)
         return
 
     import torch

@@ -65,8 +65,6 @@ __device__ __forceinline__ int table_index_d(double q, int hi) {
     return r;
 }
 
-__device__ __forceinline__ double shfl_up_d(double v, int off) { return __shfl_up(v, off, 64); }
-__device__ __forceinline__ double shfl_down_d(double v, int off) { return __shfl_down(v, off, 64); }
 __device__ __forceinline__ double readlane_d(double v, int lane) {
     const int2 w = *reinterpret_cast<int2*>(&v);
     int2 r;
@@ -85,6 +83,9 @@ __device__ __forceinline__ double dpp_d(double v) {
     r.y = __builtin_amdgcn_update_dpp(0, w.y, CTRL, ROW_MASK, 0xf, true);
     return *reinterpret_cast<double*>(&r);
 }
+
+// Lane i receives lane i+1's value (DPP wave_shl:1, no LDS round trip); lane 63 receives 0.
+__device__ __forceinline__ double shfl_next_d(double v) { return dpp_d<0x130, 0xf>(v); }
 
 // Inclusive wave64 scan without LDS round trips: row_shr 1, 2, 4, 8 scan each
 // 16-lane row, then row_bcast:15 adds row 0's total to row 1 and row 2's to
@@ -169,32 +170,99 @@ __device__ __forceinline__ void lent_write(void* p, double c, double s) {
     *reinterpret_cast<double2*>(p) = make_double2(c, s);
 }
 
-// Builds one line table {C_j, slope_j}, j = 0..n+1, from `val` (this thread's
-// pixels p0 .. p0+m-1; already rounded to the storage precision of the slope).
-// `ws` holds 4 wave sums; one barrier inside.
-template <typename T, int M>
-__device__ __forceinline__ void line_table(const double (&val)[M], int m, int p0, int n, LEnt<T>* __restrict__ tb,
-                                           double* __restrict__ ws, int tid, int lane, int wave) {
-    double run = 0;
+// Bin values y_k = post_k (S_{k+1} - S_k) from the edge sums of a wave's VPW
+// views, formed with wave shifts; RESID also writes the residual bin tables of
+// projection_t, tabs[b][view] = cumulative {C, g} of g = (y - m) * post.
+template <typename T, int VPW, int NEM, bool RESID>
+__device__ __forceinline__ void fwd_store(const DevPlan& P, const double (&S)[VPW][NEM], const int (&vw)[VPW], int grp,
+                                          int b, int mode, int lane, int wave, T* __restrict__ y,
+                                          const double* __restrict__ post, const T* __restrict__ m,
+                                          double2* __restrict__ tabs) {
+    constexpr int G = 4 * VPW;
+    const int nv = mode == MODE_Y ? P.ny : P.nx;
+    const int D = P.D;
+    const int ne = (D + 1 + 63) / 64;  // edge slots in use (<= NEM)
 #pragma unroll
-    for (int i = 0; i < M; i++)
-        if (i < m) run += val[i];
-    const double inc = wave_scan_incl(run, lane);
-    if (lane == 63) ws[wave] = inc;
-    __syncthreads();
-    double base = 0;
-    for (int q = 0; q < wave; q++) base += ws[q];
-    double pref = base + inc - run;  // sum of all pixels before p0
+    for (int v = 0; v < VPW; v++) {
+        const int idx = grp * G + wave * VPW + v;
+        if (idx >= nv) continue;  // wave-uniform
+        const int view = vw[v];
+        const bool rev = P.vp[view].rev;
+        const size_t row = (size_t)b * P.V + view;
+        const double* pr = post ? post + (size_t)view * D : nullptr;
+        const double* ev = P.edges + (size_t)view * (D + 1);
+        double2* tb = RESID ? tabs + row * (D + 2) : nullptr;
+        double carry = 0.0;
 #pragma unroll
-    for (int i = 0; i < M; i++) {
-        if (i < m) {
-            const int p = p0 + i;
-            if (p < n) lent_write(tb + p + 1, pref - (double)(p + 1) * val[i], val[i]);
-            pref += val[i];
+        for (int i = 0; i < NEM; i++) {
+            if (i >= ne) continue;  // wave-uniform
+            const int k = lane + 64 * i;
+            double nxt = shfl_next_d(S[v][i]);
+            double r0 = 0.0;
+#pragma unroll
+            for (int q = 0; q < NEM; q++)
+                if (q == i + 1) r0 = readlane_d(S[v][q], 0);
+            if (lane == 63) nxt = r0;
+            double yk = nxt - S[v][i];
+            if (pr && k < D) yk *= pr[k];
+            const int o = rev ? D - 1 - k : k;
+            if (!RESID) {
+                if (k < D) y[row * D + o] = (T)yk;
+            } else {
+                double gk = 0.0, ek = 0.0, wg = 0.0;
+                if (k < D) {
+                    if (y) y[row * D + o] = (T)yk;
+                    gk = (yk - (double)m[row * D + o]) * pr[k];
+                    ek = ev[k];
+                    wg = (ev[k + 1] - ek) * gk;
+                }
+                const double inc = wave_scan_incl(wg, lane);
+                if (k < D) tb[k + 1] = make_double2(carry + (inc - wg) - ek * gk, gk);
+                carry += readlane_d(inc, 63);
+            }
+        }
+        if (RESID && lane == 0) {
+            tb[0] = make_double2(0.0, 0.0);
+            tb[D + 1] = make_double2(carry, 0.0);
         }
     }
-    if (tid == 0) lent_write(tb, 0.0, 0.0);
-    if (tid == NT - 1) lent_write(tb + n + 1, ws[0] + ws[1] + ws[2] + ws[3], 0.0);
+}
+
+// Builds NTAB line tables {C_j, slope_j}, j = 0..n+1, table t from val[t] (this
+// thread's pixels p0 .. p0+m-1; already rounded to the storage precision of
+// the slope).  Table t starts at tb + t*tabn; `ws` holds NTAB x 4 wave sums.
+// One barrier inside, whatever NTAB.
+template <typename T, int M, int NTAB>
+__device__ __forceinline__ void line_tables(const double (&val)[NTAB][M], int m, int p0, int n, LEnt<T>* __restrict__ tb,
+                                            int tabn, double* __restrict__ ws, int tid, int lane, int wave) {
+    double run[NTAB], inc[NTAB];
+#pragma unroll
+    for (int t = 0; t < NTAB; t++) {
+        run[t] = 0;
+#pragma unroll
+        for (int i = 0; i < M; i++)
+            if (i < m) run[t] += val[t][i];
+        inc[t] = wave_scan_incl(run[t], lane);
+        if (lane == 63) ws[t * 4 + wave] = inc[t];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < NTAB; t++) {
+        LEnt<T>* tt = tb + t * tabn;
+        double base = 0;
+        for (int q = 0; q < wave; q++) base += ws[t * 4 + q];
+        double pref = base + inc[t] - run[t];  // sum of all pixels before p0
+#pragma unroll
+        for (int i = 0; i < M; i++) {
+            if (i < m) {
+                const int p = p0 + i;
+                if (p < n) lent_write(tt + p + 1, pref - (double)(p + 1) * val[t][i], val[t][i]);
+                pref += val[t][i];
+            }
+        }
+        if (tid == 0) lent_write(tt, 0.0, 0.0);
+        if (tid == NT - 1) lent_write(tt + n + 1, ws[t * 4] + ws[t * 4 + 1] + ws[t * 4 + 2] + ws[t * 4 + 3], 0.0);
+    }
 }
 
 // LDS bytes of one fwd_block with G views per block.
@@ -227,7 +295,6 @@ __device__ __forceinline__ void fwd_block(const DevPlan& P, const T* __restrict_
     const int nlines = mode == MODE_Y ? P.W : P.H;
     const int n = mode == MODE_Y ? P.H : P.W;
     const int D = P.D;
-    const int ne = (D + 1 + 63) / 64;  // edge slots in use (<= NEM)
 
     int vw[VPW];
 #pragma unroll
@@ -275,28 +342,28 @@ __device__ __forceinline__ void fwd_block(const DevPlan& P, const T* __restrict_
         double* cfl = cf + (l & 1) * G * LC_WORDS;
         if (cthread) cfl[tid] = cnext;
         if (WEIGHTED) __syncthreads();  // the weights below read cfl
+        double val[NTAB][M];
 #pragma unroll
         for (int t = 0; t < NTAB; t++) {
-            double val[M];
             if (WEIGHTED) {
                 const double w0 = cfl[t * LC_WORDS + LC_W0], w1 = cfl[t * LC_WORDS + LC_W1];
                 const double w2 = cfl[t * LC_WORDS + LC_W2], is = cfl[t * LC_WORDS + LC_INVS];
 #pragma unroll
                 for (int i = 0; i < M; i++) {
-                    val[i] = 0.0;
+                    val[t][i] = 0.0;
                     if (i < m_px && p0 + i < n) {
                         const double u = __builtin_fma(w1, (double)(p0 + i), w0);
                         const double d = __builtin_fma(u, u, w2);
                         // rounded to the slope's storage precision before the prefix sum
-                        val[i] = (double)(T)((double)xv[i] * (is * (double)__builtin_amdgcn_rcpf((float)d)));
+                        val[t][i] = (double)(T)((double)xv[i] * (is * (double)__builtin_amdgcn_rcpf((float)d)));
                     }
                 }
             } else {
 #pragma unroll
-                for (int i = 0; i < M; i++) val[i] = (i < m_px && p0 + i < n) ? (double)xv[i] : 0.0;
+                for (int i = 0; i < M; i++) val[t][i] = (i < m_px && p0 + i < n) ? (double)xv[i] : 0.0;
             }
-            line_table<T, M>(val, m_px, p0, n, tb + t * tabn, ws + t * 4, tid, lane, wave);
         }
+        line_tables<T, M, NTAB>(val, m_px, p0, n, tb, tabn, ws, tid, lane, wave);
         if (l + 1 < nlines) {
             const T* nb = base + (size_t)(l + 1) * n;
 #pragma unroll
@@ -323,51 +390,101 @@ __device__ __forceinline__ void fwd_block(const DevPlan& P, const T* __restrict_
         }
     }
 
-    // Bin values from neighbouring edge sums, per wave.
+    fwd_store<T, VPW, NEM, RESID>(P, S, vw, grp, b, mode, lane, wave, y, post, m, tabs);
+}
+
+// Entries of a wave-private line table sit at j + j/8 (one 16-byte pad every 8):
+// lane L writes entries 8L+1 .. 8L+8, and with the pad consecutive lanes land
+// 9 entries apart, so each ds_write_b128 of 16 lanes covers all 64 banks.
+__host__ __device__ inline int wtab_slots(int n) { return n + 2 + (n + 2) / 8 + 1; }
+__device__ __forceinline__ int wtab_slot(int j) { return j + (j >> 3); }
+
+// LDS bytes of one fwd_block_w: one wave-private table per wave.
+__host__ __device__ inline size_t fwd_w_block_lds(int nmax) { return (size_t)4 * wtab_slots(nmax) * 16; }
+
+// Weighted line -> bin work item (backprojection_t and its exact adjoint form)
+// for lines of at most 64*PPL pixels: G = 4 views per block, one per wave, and
+// each wave builds the weighted cumulative table of its own view for every line
+// by itself: lane L owns pixels L*PPL .. L*PPL+PPL-1 (one serial sum and one
+// wave scan per line) and writes them into its wave's private table, so no
+// barrier is needed anywhere.  The line's pixels and the view's LineCoef words
+// are loaded one line ahead into registers.  Same arithmetic as fwd_block with
+// WEIGHTED (pixel weight with a fp32 reciprocal, weighted value rounded to the
+// storage type before the prefix sum).
+template <typename T, int NEM, int PPL>
+__device__ __forceinline__ void fwd_block_w(const DevPlan& P, const T* __restrict__ src, T* __restrict__ y, int b,
+                                            int mode, int grp, const double* __restrict__ post,
+                                            unsigned char* __restrict__ smem) {
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int* vlist = mode == MODE_Y ? P.yviews : P.xviews;
+    const int nv = mode == MODE_Y ? P.ny : P.nx;
+    const int nlines = mode == MODE_Y ? P.W : P.H;
+    const int n = mode == MODE_Y ? P.H : P.W;
+    const int D = P.D;
+
+    int vw[1];
+    {
+        const int idx = grp * 4 + wave;
+        vw[0] = vlist[idx < nv ? idx : nv - 1];
+    }
+    double e[1][NEM], S[1][NEM];
 #pragma unroll
-    for (int v = 0; v < VPW; v++) {
-        const int idx = grp * G + wave * VPW + v;
-        if (idx >= nv) continue;  // wave-uniform
-        const int view = vw[v];
-        const bool rev = P.vp[view].rev;
-        const size_t row = (size_t)b * P.V + view;
-        const double* pr = post ? post + (size_t)view * D : nullptr;
-        const double* ev = P.edges + (size_t)view * (D + 1);
-        double2* tb = RESID ? tabs + row * (D + 2) : nullptr;
-        double carry = 0.0;
+    for (int i = 0; i < NEM; i++) {
+        const int k = lane + 64 * i;
+        e[0][i] = P.edges[(size_t)vw[0] * (D + 1) + (k <= D ? k : D)];
+        S[0][i] = 0.0;
+    }
+
+    double2* tb = reinterpret_cast<double2*>(smem) + (size_t)wave * wtab_slots(n);
+    const T* base = src + (size_t)b * nlines * n;
+    const int p0 = lane * PPL;
+    const int hi = n + 1;
+    // LineCoef word `lane` (lanes 0..7) of this wave's view, one line ahead
+    const double* crow = reinterpret_cast<const double*>(P.lines + (size_t)vw[0] * P.max_lines) + (lane & 7);
+    auto load_line = [&](int l, T (&xv)[PPL], double& cw) {
+        const T* row = base + (size_t)l * n;
+#pragma unroll
+        for (int k = 0; k < PPL; k++) xv[k] = p0 + k < n ? row[p0 + k] : T(0);
+        cw = crow[(size_t)l * LC_WORDS];
+    };
+    T xv[PPL];
+    double cw;
+    load_line(0, xv, cw);
+
+    for (int l = 0; l < nlines; l++) {
+        const double s = readlane_d(cw, LC_S), A = readlane_d(cw, LC_A), Bc = readlane_d(cw, LC_B);
+        const double w0 = readlane_d(cw, LC_W0), w1 = readlane_d(cw, LC_W1);
+        const double w2 = readlane_d(cw, LC_W2), is = readlane_d(cw, LC_INVS);
+        double val[PPL], run = 0.0;
+#pragma unroll
+        for (int k = 0; k < PPL; k++) {
+            const double u = __builtin_fma(w1, (double)(p0 + k), w0);
+            const double d = __builtin_fma(u, u, w2);
+            val[k] = p0 + k < n ? (double)(T)((double)xv[k] * (is * (double)__builtin_amdgcn_rcpf((float)d))) : 0.0;
+            run += val[k];
+        }
+        if (l + 1 < nlines) load_line(l + 1, xv, cw);
+        // Table of this line: the wave's reads of the previous line's table were
+        // consumed above (in-order LDS within a wave), so it can be overwritten.
+        const double inc = wave_scan_incl(run, lane);
+        double pref = inc - run;
+#pragma unroll
+        for (int k = 0; k < PPL; k++) {
+            const int p = p0 + k;
+            if (p < n) tb[wtab_slot(p + 1)] = make_double2(__builtin_fma(-(double)(p + 1), val[k], pref), val[k]);
+            pref += val[k];
+        }
+        if (lane == 0) tb[0] = make_double2(0.0, 0.0);
+        if (lane == 63) tb[wtab_slot(n + 1)] = make_double2(inc, 0.0);
 #pragma unroll
         for (int i = 0; i < NEM; i++) {
-            if (i >= ne) continue;  // wave-uniform
-            const int k = lane + 64 * i;
-            double nxt = shfl_down_d(S[v][i], 1);
-            double r0 = 0.0;
-#pragma unroll
-            for (int q = 0; q < NEM; q++)
-                if (q == i + 1) r0 = readlane_d(S[v][q], 0);
-            if (lane == 63) nxt = r0;
-            double yk = nxt - S[v][i];
-            if (pr && k < D) yk *= pr[k];
-            const int o = rev ? D - 1 - k : k;
-            if (!RESID) {
-                if (k < D) y[row * D + o] = (T)yk;
-            } else {
-                double gk = 0.0, ek = 0.0, wg = 0.0;
-                if (k < D) {
-                    if (y) y[row * D + o] = (T)yk;
-                    gk = (yk - (double)m[row * D + o]) * pr[k];
-                    ek = ev[k];
-                    wg = (ev[k + 1] - ek) * gk;
-                }
-                const double inc = wave_scan_incl(wg, lane);
-                if (k < D) tb[k + 1] = make_double2(carry + (inc - wg) - ek * gk, gk);
-                carry += readlane_d(inc, 63);
-            }
-        }
-        if (RESID && lane == 0) {
-            tb[0] = make_double2(0.0, 0.0);
-            tb[D + 1] = make_double2(carry, 0.0);
+            const double tau = __builtin_fma(A, e[0][i], Bc);
+            const double2 ent = tb[wtab_slot(table_index_d(tau, hi))];
+            S[0][i] = __builtin_fma(s, __builtin_fma(tau, ent.y, ent.x), S[0][i]);
         }
     }
+    fwd_store<T, 1, NEM, false>(P, S, vw, grp, b, mode, lane, wave, y, post, nullptr, nullptr);
 }
 
 // ---------------------------------------------------------------------------
@@ -523,7 +640,7 @@ __device__ __forceinline__ void adj_block(const DevPlan& P, const double2* __res
                 for (int i = 0; i < NPE; i++) {
                     const double gi = eval_G<GEO>(t, a, s, lane + 64 * i, lu, hi);
                     if (i > 0) {
-                        double right = shfl_down_d(gprev, 1);
+                        double right = shfl_next_d(gprev);
                         const double r0 = readlane_d(gi, 0);
                         if (lane == 63) right = r0;
                         const int p = lane + 64 * (i - 1);
@@ -534,7 +651,7 @@ __device__ __forceinline__ void adj_block(const DevPlan& P, const double2* __res
                 // pixels of the last slot group (their right edges are in the same group;
                 // lane 63's pixel is past the line since 64 * NPE > n)
                 {
-                    const double right = shfl_down_d(gprev, 1);
+                    const double right = shfl_next_d(gprev);
                     const int p = lane + 64 * (NPE - 1);
                     acc[q][NPE - 1] += p < n ? wgt(p) * (right - gprev) : 0.0;
                 }
@@ -557,7 +674,7 @@ __device__ __forceinline__ void adj_block(const DevPlan& P, const double2* __res
             const int p = lane + 64 * i;
             double val;
             if (!WEIGHTED) {
-                double right = shfl_down_d(acc[q][i], 1);
+                double right = shfl_next_d(acc[q][i]);
                 double r0 = 0.0;
 #pragma unroll
                 for (int k = 0; k < NPE; k++)

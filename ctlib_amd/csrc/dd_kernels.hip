@@ -151,7 +151,7 @@ __global__ __launch_bounds__(256) void k_bprj_sv(DevPlan P, const double2* __res
             for (int i = 0; i < NPE; i++) g[i] = eval_G<GEO>(t, c.a, c.s, lane + 64 * i, lu, hi);
 #pragma unroll
             for (int i = 0; i < NPE; i++) {
-                double right = shfl_down_d(g[i], 1);
+                double right = shfl_next_d(g[i]);
                 if (i + 1 < NPE) {
                     const double r0 = readlane_d(g[i + 1], 0);
                     if (lane == 63) right = r0;
@@ -280,6 +280,7 @@ hipError_t launch_fwd(const DevPlan& P, int dtype, const void* srcx, const void*
                       const double* post, FwdShape shape, hipStream_t s) {
     if (B <= 0 || P.nx + P.ny == 0) return hipSuccess;
     if ((P.nx && P.W > NT * 8) || (P.ny && P.H > NT * 8) || P.D > 2046) return hipErrorInvalidValue;
+    if (weighted && fwd_w_supported(P)) return launch_fwd_w(P, dtype, srcx, srcy, y, B, post, s);
     FwdArgs a{srcx, srcy, y, B, post, nullptr, nullptr};
     if (dtype == DT_F32) return fwd_dispatch<float>(P, a, weighted, false, shape.G, s);
     return fwd_dispatch<double>(P, a, weighted, false, shape.G, s);

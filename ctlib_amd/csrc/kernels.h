@@ -50,6 +50,12 @@ size_t fwd_lds_bytes(const DevPlan& P, bool weighted, int G);
 hipError_t launch_fwd_resid(const DevPlan& P, int dtype, const void* srcx, const void* srcy, void* y, const void* m,
                             double2* tabs, int B, FwdShape shape, hipStream_t s);
 
+// Weighted line -> bin operator (backprojection_t; post may be null) with wave-private line
+// tables, for lines of at most 1024 pixels (fwd_w_supported); launch_fwd dispatches to it.
+bool fwd_w_supported(const DevPlan& P);
+hipError_t launch_fwd_w(const DevPlan& P, int dtype, const void* srcx, const void* srcy, void* y, int B,
+                        const double* post, hipStream_t s);
+
 // Per (b, view) cumulative-integral tables {C_j, g_j}, j = 0..D+1, of g_k = (y - m)[o(k)] * pre[k] * scale.
 hipError_t launch_bin_tables(const DevPlan& P, int in_dtype, const void* y, const void* m, const double* pre,
                              double scale, double2* tabs, int B, hipStream_t s);
