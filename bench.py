@@ -207,6 +207,52 @@ def workload_ops(args):
     return out
 
 
+def workload_laplacian(args):
+    """laplacian (kNN graph weights) on N=16384 points of dimension 49 (7x7 patches), k=16, fp32.
+    Compute-bound: 2*N^2*d flops of distances (VALU FMA chains, no N x N matrix in memory)."""
+    import torch
+
+    import ctlib_amd
+    from oracle import laplacian as L
+
+    dev = torch.device("cuda:0")
+    N, d, k = 16384, 49, 16
+    x = torch.rand(N, d, generator=torch.Generator().manual_seed(7)).to(dev)
+    for _ in range(args.warmup):
+        ctlib_amd.laplacian(x, k)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        W = ctlib_amd.laplacian(x, k)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    flops = 2.0 * N * N * d
+    out = {
+        "metric": "laplacian calls/s, N=16384 points, d=49, k=16 -> sparse (N, N)",
+        "value": round(1e3 / ms, 3), "unit": "calls/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(ms, 4), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "kNN graph weights of 16384 random points in [0,1]^49, k=16", "nnz": int(W._nnz())},
+        "roofline": {"bound": "valu", "achieved": round(flops / (ms * 1e-3) / 1e12, 2), "peak": 157.3,
+                     "unit": "TFLOP/s", "frac": round(flops / (ms * 1e-3) / 157.3e12, 4), "traffic": None,
+                     "note": "distance flops 2*N^2*d over the whole call; peak = fp32 vector with packed FMA"},
+    }
+    if not args.no_cpu:
+        # FP64 numpy restatement: distances and selection for 256 query rows, scaled to N rows
+        xd = x.double().cpu().numpy()
+        t0 = time.perf_counter()
+        n2 = (xd * xd).sum(1)
+        q = xd[:256]
+        dd = n2[:256, None] + n2[None, :] - 2.0 * (q @ xd.T)
+        idx = np.arange(N)
+        for i in range(256):
+            np.lexsort((idx, dd[i]))[:k]
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": round(1.0 / (dt * N / 256), 5), "unit": "calls/s", "cores": 1,
+                               "kind": "port", "sample": f"numpy FP64 restatement, 256 of {N} rows ({dt:.2f} s), scaled"}
+    return out
+
+
 def workload_sv(args):
     """backprojection_sv (per-view backprojection) on fan-ED 512^2, 736 views x 736 bins, B=8.
 
@@ -328,10 +374,10 @@ def main():
     ap.add_argument("--paths", default="plain,autograd,graph")
     ap.add_argument("--calib-copy", action="store_true",
                     help="run one 256 MiB device copy first (reference kernel for HBM counter scaling)")
-    ap.add_argument("--workload", default="metric", choices=("metric", "hetero", "unroll", "sv", "ops"),
+    ap.add_argument("--workload", default="metric", choices=("metric", "hetero", "unroll", "sv", "ops", "laplacian"),
                     help="metric: the headline fan-ED B=32 step (default); hetero: 64-slice batch; "
                          "unroll: 50-iteration checkpointed reconstruction; sv: backprojection_sv; "
-                         "ops: every public operator")
+                         "ops: every public operator; laplacian: kNN graph weights")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; the projector shards over slices / views "
@@ -350,6 +396,9 @@ def main():
         return
     if args.workload == "ops":
         print(json.dumps(workload_ops(args)), flush=True)
+        return
+    if args.workload == "laplacian":
+        print(json.dumps(workload_laplacian(args)), flush=True)
         return
 
     import torch

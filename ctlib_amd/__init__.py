@@ -9,6 +9,7 @@ Drop-in for the reference module `ctlib` (ctlib.cpp:106-114):
     sino = ctlib.backprojection_t(image, options)    # transpose of backprojection (reference form)
     img  = ctlib.fbp(sino, options)                  # RL-filtered backprojection
     imgs = ctlib.backprojection_sv(sino, options)    # per-view backprojection, B x V x H x W (fan-ED)
+    W    = ctlib.laplacian(points, k)                # kNN graph weights, sparse N x N
 
 `options` is the reference's vector: 12 entries for fan beam
 [views, dets, width, height, dImg, dDet, Ang0, dAng, s2r, d2r, binshift, scan_type]
@@ -100,8 +101,15 @@ def backprojection_sv(projection, options):
 
 
 def laplacian(input, k):
-    """kNN graph Laplacian of the MAGIC network (ctlib.cpp:101-104): out of scope, not implemented."""
-    raise NotImplementedError("laplacian is not part of the ctlib_amd projector library")
+    """kNN graph weights of the MAGIC network (ctlib.cpp:101-104, laplacian_cuda_kernel.cu:46-128).
+
+    input: N x d points (float32 or float64, contiguous, on the GPU).  Returns the sparse COO
+    (N, N) float32 tensor whose row i holds, at the k points nearest to point i by squared
+    distance (itself included), exp(-dist / median) normalised to sum 1, median being the lower
+    median of all N*k selected distances.  The N x N distance matrix is never formed.
+    Supported: d <= 128 and k <= 64 with d + 2k <= 192 (float32) or 2d + 3k <= 192 (float64).
+    """
+    return _C.laplacian(input, int(k))
 
 
 __all__ = ["projection", "projection_t", "backprojection", "backprojection_t", "fbp", "backprojection_t_true",

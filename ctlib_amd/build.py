@@ -65,6 +65,7 @@ def build(arch="gfx950", verbose=False, with_torch=True):
         "dd_fwd_w.o": [HIPCC, f"--offload-arch={arch}", *HIP_FLAGS, "-c", os.path.join(CSRC, "dd_fwd_w.hip")],
         "ramp_mfma.o": [HIPCC, f"--offload-arch={arch}", *HIP_FLAGS, "-c", os.path.join(CSRC, "ramp_mfma.hip")],
         "batch.o": [HIPCC, f"--offload-arch={arch}", *HIP_FLAGS, "-c", os.path.join(CSRC, "batch.hip")],
+        "laplacian.o": [HIPCC, f"--offload-arch={arch}", *HIP_FLAGS, "-c", os.path.join(CSRC, "laplacian.hip")],
     }
     jobs = []
     for name, cmd in objs.items():

@@ -537,6 +537,28 @@ int ctlib_run(const ctlib_plan* p, int op, int dtype, const void* in, void* out,
     return CTLIB_OK;
 }
 
+size_t ctlib_laplacian_workspace_bytes(int64_t n, int d, int k, int dtype) {
+    (void)d;
+    if (n <= 0 || k <= 0) return 0;
+    return laplacian_ws_bytes(n, k, dtype);
+}
+
+int ctlib_laplacian(const void* x, int64_t n, int d, int k, int dtype, int64_t* cols, float* vals, void* ws,
+                    size_t ws_bytes, void* stream) {
+    if (dtype != CTLIB_F32 && dtype != CTLIB_F64) return fail(CTLIB_ERR_UNSUPPORTED_DTYPE, "dtype must be f32 or f64");
+    if (n < 1 || n > 0x7fffffff || d < 1) return fail(CTLIB_ERR_INVALID_ARG, "need 1 <= points < 2^31, dimension >= 1");
+    if (k < 1 || k > n) return fail(CTLIB_ERR_INVALID_ARG, "k must be in [1, number of points]");
+    if (!x || !cols || !vals) return fail(CTLIB_ERR_INVALID_ARG, "NULL pointer");
+    if (!laplacian_supported(d, k, dtype))
+        return fail(CTLIB_ERR_INVALID_ARG, "supported: dimension <= 128, k <= 64, dimension + 2k <= 192 (fp32) or "
+                                           "2 dimension + 3k <= 192 (fp64)");
+    if (!ws || ws_bytes < ctlib_laplacian_workspace_bytes(n, d, k, dtype))
+        return fail(CTLIB_ERR_WORKSPACE, "workspace smaller than ctlib_laplacian_workspace_bytes()");
+    hipError_t e = launch_laplacian(x, n, d, k, dtype, cols, vals, ws, static_cast<hipStream_t>(stream));
+    if (e != hipSuccess) return hip_fail(e, "laplacian launch");
+    return CTLIB_OK;
+}
+
 int ctlib_timing_enable(int on) {
     std::lock_guard<std::mutex> lock(g_tmu);
     g_timing = on != 0;

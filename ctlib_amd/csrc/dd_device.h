@@ -630,7 +630,15 @@ __device__ __forceinline__ void adj_block(const DevPlan& P, const double2* __res
                 const double w0 = cf[lq * LC_WORDS + LC_W0], w1 = cf[lq * LC_WORDS + LC_W1];
                 const double w2 = cf[lq * LC_WORDS + LC_W2];
                 const double ws = cf[lq * LC_WORDS + LC_INVS] * wscale;
+                // Pixel weight ws / ((w0 + w1 p)^2 + w2).  fp32 storage: evaluated in fp32
+                // (relative error ~2e-7, below the output's rounding); fp64 storage: u and
+                // d in fp64, fp32 reciprocal.
+                const float w0f = (float)w0, w1f = (float)w1, w2f = (float)w2, wsf = (float)ws;
                 auto wgt = [&](int p) {
+                    if (sizeof(T) == 4) {
+                        const float u = __builtin_fmaf(w1f, (float)p, w0f);
+                        return (double)(wsf * __builtin_amdgcn_rcpf(__builtin_fmaf(u, u, w2f)));
+                    }
                     const double u = __builtin_fma(w1, (double)p, w0);
                     const double d = __builtin_fma(u, u, w2);
                     return ws * (double)__builtin_amdgcn_rcpf((float)d);

@@ -164,6 +164,31 @@ torch::Tensor backprojection_sv(torch::Tensor projection, torch::Tensor options,
     }
     return run_op(projection, options, CTLIB_OP_BPRJ_SV, SINO_TO_VIEWS, "projection");
 }
+// ctlib.cpp:101-104, laplacian_cuda_kernel.cu:46-128: kNN graph weights as a sparse COO
+// (N, N) float32 tensor, row i holding its k nearest points (itself included).
+torch::Tensor laplacian(torch::Tensor input, int64_t k) {
+    TORCH_CHECK(input.is_cuda(), "input must be a GPU (HIP) tensor");
+    TORCH_CHECK(input.is_contiguous(), "input must be contiguous");
+    TORCH_CHECK(input.dim() == 2, "input must be 2-D (points x dimension), got ", input.dim(), "-D");
+    const int dt = dtype_code(input);
+    const int64_t n = input.size(0);
+    const int d = (int)input.size(1);
+    const int device = input.get_device();
+    c10::hip::HIPGuard guard((c10::DeviceIndex)device);
+    if (k < 1 || k > n) throw pybind11::value_error("k must be in [1, number of points]");
+    torch::Tensor cols = torch::empty({n, k}, input.options().dtype(torch::kInt64));
+    torch::Tensor vals = torch::empty({n, k}, input.options().dtype(torch::kFloat32));
+    const size_t wsb = ctlib_laplacian_workspace_bytes(n, d, (int)k, dt);
+    torch::Tensor ws = torch::empty({(int64_t)wsb}, input.options().dtype(torch::kUInt8));
+    hipStream_t s = c10::hip::getCurrentHIPStream((c10::DeviceIndex)device).stream();
+    int rc = ctlib_laplacian(input.data_ptr(), n, d, (int)k, dt, cols.data_ptr<int64_t>(), vals.data_ptr<float>(),
+                             ws.data_ptr(), wsb, (void*)s);
+    if (rc) raise_status(rc, "laplacian");
+    torch::Tensor rows = torch::arange(n, cols.options()).unsqueeze(1).expand({n, k});
+    torch::Tensor idx = torch::stack({rows.reshape({-1}), cols.reshape({-1})});
+    return torch::sparse_coo_tensor(idx, vals.reshape({-1}), {n, n}, vals.options().layout(torch::kSparse));
+}
+
 // Exact adjoints used by autograd (no reference counterpart).
 torch::Tensor backprojection_t_true(torch::Tensor image, torch::Tensor options) {
     return run_op(image, options, CTLIB_OP_BPRJ_T_TRUE, IMG_TO_SINO, "image");
@@ -355,6 +380,7 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m) {
     m.def("backprojection_sv", &backprojection_sv, "Per-view backprojection, [B,V,H,W] (HIP)",
           pybind11::arg("projection"), pybind11::arg("options"), pybind11::arg("any_scan_type") = false);
     m.def("residual_step", &residual_step, "(A x, A^T (A x - m)) in one call (HIP)");
+    m.def("laplacian", &laplacian, "kNN graph weights as a sparse (N, N) tensor (HIP)");
     m.def("parse_options", &parse_options, "Parse an options vector");
     m.def("workspace_bytes", &workspace_bytes, "Scratch bytes of one op");
     m.def("clear_plan_cache", &clear_plan_cache, "Drop all cached geometry plans");

@@ -140,6 +140,18 @@ int ctlib_batch_info(const ctlib_batch* batch, int* nitems, int* blocks_per_cu, 
 int ctlib_batch_check(const ctlib_batch* batch, void* stream);
 const char* ctlib_batch_last_error(void);
 
+/* kNN graph weights of the MAGIC network (laplacian_cuda_kernel.cu:46-128,
+ * ctlib.cpp:101-104): for n points x [n][d] (dtype f32 or f64, device memory),
+ * the k nearest points of each point by squared distance (itself included),
+ * cols [n][k] int64 in ascending distance (ties by column), and their weights
+ * exp(-dist / median) normalised per point, vals [n][k] float32, where median
+ * is the lower median of all n*k selected distances.  Row i of the sparse
+ * matrix holds cols[i] / vals[i].  Supported: d <= 128, k <= 64 with
+ * d + 2k <= 192 (f32) or 2d + 3k <= 192 (f64).  Asynchronous on `stream`. */
+size_t ctlib_laplacian_workspace_bytes(int64_t n, int d, int k, int dtype);
+int ctlib_laplacian(const void* x, int64_t n, int d, int k, int dtype, int64_t* cols, float* vals, void* ws,
+                    size_t ws_bytes, void* stream);
+
 /* Kernel timing.  While enabled, ctlib_run / ctlib_run_residual record a HIP
  * event pair on the launch stream around each launch, grouped in classes:
  * 0 line->bin DD kernel, 1 bin->pixel DD kernel, 2 ramp GEMM, 3 auxiliary

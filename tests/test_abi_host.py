@@ -148,3 +148,22 @@ def test_fan_ea_atan_polynomial(dDet, expect_poly):
     assert np.max(np.abs(p - np.arctan(t))) <= 5e-8
     # monotone on [0, T], so clamped rays beyond T stay beyond the last edge
     assert np.all(np.diff(p) > 0)
+
+
+def test_laplacian_argument_checks_without_gpu():
+    """ctlib_laplacian validates its arguments before any device work."""
+    import ctypes
+    lib = abi.load()
+    lib.ctlib_laplacian.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                    ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t,
+                                    ctypes.c_void_p]
+    lib.ctlib_laplacian_workspace_bytes.restype = ctypes.c_size_t
+    lib.ctlib_laplacian_workspace_bytes.argtypes = [ctypes.c_int64, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    fake = ctypes.c_void_p(0x1000)
+    # k > n, k < 1, unsupported dimension, bad dtype, missing workspace
+    assert lib.ctlib_laplacian(fake, 10, 4, 11, 0, fake, fake, fake, 1 << 20, None) == 1
+    assert lib.ctlib_laplacian(fake, 10, 4, 0, 0, fake, fake, fake, 1 << 20, None) == 1
+    assert lib.ctlib_laplacian(fake, 10, 300, 4, 0, fake, fake, fake, 1 << 20, None) == 1
+    assert lib.ctlib_laplacian(fake, 10, 4, 4, 7, fake, fake, fake, 1 << 20, None) == 5
+    assert lib.ctlib_laplacian(fake, 10, 4, 4, 0, fake, fake, None, 0, None) == 6
+    assert lib.ctlib_laplacian_workspace_bytes(1000, 16, 8, 0) >= 1000 * 4 + 1000 * 8 * 4

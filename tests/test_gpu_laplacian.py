@@ -1,0 +1,92 @@
+"""laplacian (kNN graph weights, laplacian_cuda_kernel.cu:46-128) on the GPU against the
+numpy oracle.  fp32 distances differ from the oracle's fp64 ones by rounding, so a row whose
+k-th and (k+1)-th oracle distances are closer than a tie band may legitimately pick a different
+k-th point; every other row must select the same columns, and the weights of the selected
+columns must agree to 1e-5 relative L2."""
+import numpy as np
+import pytest
+import torch
+
+import ctlib_amd
+from oracle import laplacian as L
+
+pytestmark = pytest.mark.gpu
+
+
+def dense(W):
+    W = W.coalesce()
+    return W.indices().cpu().numpy(), W.values().cpu().numpy()
+
+
+def check(x, k, tie_band):
+    N = x.shape[0]
+    W = ctlib_amd.laplacian(x, k)
+    assert W.is_sparse and W.shape == (N, N) and W.dtype == torch.float32 and W.device == x.device
+    assert W._nnz() == N * k
+    idx = W._indices().cpu().numpy()
+    vals = W._values().cpu().numpy().reshape(N, k)
+    rows = idx[0].reshape(N, k)
+    cols = idx[1].reshape(N, k)
+    assert (rows == np.arange(N)[:, None]).all()
+    xd = x.double().cpu().numpy()
+    d = L.distances(xd)
+    oc, ov = L.laplacian(xd, k, dist=d)
+    ds = np.sort(d, axis=1)
+    kth = ds[:, k - 1]
+    nxt = ds[:, k] if k < N else np.full(N, np.inf)
+    clear = (nxt - kth) > tie_band * np.maximum(1.0, np.abs(kth))
+    same = np.array([set(cols[i]) == set(oc[i]) for i in range(N)])
+    assert same[clear].all(), np.nonzero(~same & clear)[0][:10]
+    assert (~same).sum() <= max(2, N // 100)
+    # weights by column for the rows that agree
+    gv = np.zeros_like(ov)
+    for i in np.nonzero(same)[0]:
+        m = dict(zip(cols[i].tolist(), vals[i].tolist()))
+        gv[i] = [m[c] for c in oc[i]]
+    err = np.linalg.norm((gv - ov)[same]) / np.linalg.norm(ov[same])
+    assert err <= 1e-5, err
+    # ascending distance within each row (ties by column)
+    gd = np.take_along_axis(d, cols, 1)
+    assert (np.diff(gd, axis=1) >= -tie_band * np.maximum(1.0, np.abs(gd[:, 1:]))).all()
+    return W
+
+
+@pytest.mark.parametrize("N, d, k, dtype", [
+    (1000, 16, 8, torch.float32), (777, 49, 16, torch.float32), (1500, 100, 32, torch.float32),
+    (600, 3, 64, torch.float32), (513, 128, 8, torch.float32), (64, 9, 64, torch.float32),
+    (900, 16, 8, torch.float64), (700, 49, 16, torch.float64), (300, 30, 32, torch.float64),
+])
+def test_against_oracle(dev, N, d, k, dtype):
+    x = torch.rand(N, d, generator=torch.Generator().manual_seed(N + d + k), dtype=dtype).to(dev)
+    print(f"[case] laplacian N={N} d={d} k={k} {dtype}", flush=True)
+    check(x, k, 1e-5 if dtype == torch.float32 else 1e-12)
+
+
+def test_duplicates_and_determinism(dev):
+    g = torch.Generator().manual_seed(4)
+    base = torch.rand(200, 12, generator=g)
+    x = torch.cat([base, base[:50]]).contiguous().to(dev)   # 50 exact duplicates
+    print("[case] laplacian duplicates", flush=True)
+    W1 = ctlib_amd.laplacian(x, 4)
+    W2 = ctlib_amd.laplacian(x, 4)
+    assert torch.equal(W1._indices(), W2._indices()) and torch.equal(W1._values(), W2._values())
+    cols = W1._indices()[1].reshape(250, 4).cpu()
+    # a duplicated point lists itself and its twin first, lower column first
+    for i in range(50):
+        assert cols[i, :2].tolist() == [i, 200 + i]
+        assert cols[200 + i, :2].tolist() == [i, 200 + i]
+
+
+def test_errors(dev):
+    x = torch.rand(10, 4, device=dev)
+    print("[case] laplacian errors", flush=True)
+    with pytest.raises(ValueError):
+        ctlib_amd.laplacian(x, 0)
+    with pytest.raises(ValueError):
+        ctlib_amd.laplacian(x, 11)
+    with pytest.raises(ValueError):
+        ctlib_amd.laplacian(torch.rand(10, 200, device=dev), 4)      # dimension above the supported 128
+    with pytest.raises(RuntimeError):
+        ctlib_amd.laplacian(torch.rand(10, 4), 4)                     # CPU tensor
+    with pytest.raises(TypeError):
+        ctlib_amd.laplacian(x.half(), 4)
