@@ -538,9 +538,8 @@ int ctlib_run(const ctlib_plan* p, int op, int dtype, const void* in, void* out,
 }
 
 size_t ctlib_laplacian_workspace_bytes(int64_t n, int d, int k, int dtype) {
-    (void)d;
-    if (n <= 0 || k <= 0) return 0;
-    return laplacian_ws_bytes(n, k, dtype);
+    if (n <= 0 || k <= 0 || d <= 0) return 0;
+    return laplacian_ws_bytes(n, d, k, dtype == CTLIB_F64 ? CTLIB_F64 : CTLIB_F32);
 }
 
 int ctlib_laplacian(const void* x, int64_t n, int d, int k, int dtype, int64_t* cols, float* vals, void* ws,

@@ -83,7 +83,7 @@ hipError_t launch_ramp(const DevPlan& P, int in_dtype, const void* in, int out_d
 // kNN graph weights (laplacian): cols [n][k] int64, vals [n][k] float32 from points x [n][d] (dtype DT_F32 or
 // DT_F64 as CTLIB_F32 / CTLIB_F64); ws of laplacian_ws_bytes.  Supported (d, k): laplacian_supported.
 bool laplacian_supported(int d, int k, int dtype);
-size_t laplacian_ws_bytes(int64_t n, int k, int dtype);
+size_t laplacian_ws_bytes(int64_t n, int d, int k, int dtype);
 hipError_t launch_laplacian(const void* x, int64_t n, int d, int k, int dtype, int64_t* cols, float* vals, void* ws,
                             hipStream_t s);
 

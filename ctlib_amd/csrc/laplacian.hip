@@ -3,16 +3,22 @@
 //
 //   k_lap_norm     ||x_i||^2, sequential over the dimension (as :9-22)
 //   k_lap_knn      one query point per thread, its coordinates in registers; the
-//                  candidates stream through LDS in tiles of 64 points, each
-//                  distance ||x_q||^2 + ||x_c||^2 - 2 x_q.x_c is formed with a
-//                  sequential FMA chain and offered to the thread's sorted list
-//                  of the KMAX smallest (value, column) pairs held in registers
-//   k_lap_median   lower median of the N*k selected distances (torch.median, :115)
-//                  by an 8-bit radix select in one block (LDS histograms)
+//                  candidates of one slice of the point set stream through LDS in
+//                  tiles of 64 points, each distance ||x_q||^2 + ||x_c||^2 - 2 x_q.x_c
+//                  is formed with a sequential FMA chain and offered to the thread's
+//                  sorted list of the KMAX smallest (value, column) pairs held in
+//                  registers.  The grid is (query blocks) x (candidate slices), so a
+//                  few thousand points still fill the 256 CUs.
+//   k_lap_merge    per query: the slices' sorted lists -> the k smallest
+//   k_lap_hist / k_lap_pick
+//                  lower median of the N*k selected distances (torch.median, :115):
+//                  8-bit radix select, each pass a grid-wide histogram (LDS, then
+//                  global atomics) and a one-block digit pick
 //   k_lap_weights  exp(-v / median), row-normalised, stored as float32 (:116-126)
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <algorithm>
 #include <type_traits>
 
 #include "../../include/ctlib_abi.h"
@@ -21,8 +27,9 @@
 namespace ctlib {
 namespace {
 
-constexpr int LAP_NT = 256;  // threads (query points) per k_lap_knn block
-constexpr int LAP_TC = 64;   // candidate points per LDS tile
+constexpr int LAP_NT = 256;   // threads (query points) per k_lap_knn block
+constexpr int LAP_TC = 64;    // candidate points per LDS tile
+constexpr int LAP_GRID = 2048;  // blocks the (query, slice) grid aims for
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_lap_norm(const T* __restrict__ x, int64_t n, int d, T* __restrict__ norm) {
@@ -40,9 +47,28 @@ __device__ __forceinline__ bool lap_before(T v, int c, T w, int e) {
     return v < w || (v == w && c < e);
 }
 
+// Offer (v, c) to a sorted register list of KMAX entries (static indices only).
+template <typename T, int KMAX>
+__device__ __forceinline__ void lap_offer(T (&lv)[KMAX], int (&li)[KMAX], T v, int c) {
+    if (!lap_before(v, c, lv[KMAX - 1], li[KMAX - 1])) return;
+#pragma unroll
+    for (int j = 0; j < KMAX; j++) {
+        const bool lt = lap_before(v, c, lv[j], li[j]);
+        const T tv = lt ? lv[j] : v;
+        const int tc = lt ? li[j] : c;
+        lv[j] = lt ? v : lv[j];
+        li[j] = lt ? c : li[j];
+        v = tv;
+        c = tc;
+    }
+}
+
+// Block (bx, by): queries bx*256 .. +255 against candidate slice by (tiles
+// [by*tps, (by+1)*tps) of 64 points).  Writes the slice's sorted KMAX-list of
+// every query to part[by][q][0..KMAX).
 template <typename T, int DMAX, int KMAX>
 __global__ __launch_bounds__(256) void k_lap_knn(const T* __restrict__ x, const T* __restrict__ norm, int n, int d,
-                                                 int k, int64_t* __restrict__ cols, T* __restrict__ vals) {
+                                                 int tps, T* __restrict__ pv, int* __restrict__ pc) {
     __shared__ T ct[LAP_TC][DMAX];
     __shared__ T cn[LAP_TC];
     const int tid = threadIdx.x;
@@ -59,8 +85,10 @@ __global__ __launch_bounds__(256) void k_lap_knn(const T* __restrict__ x, const 
         lv[j] = __builtin_inf();
         li[j] = 0x7fffffff;
     }
-    for (int c0 = 0; c0 < n; c0 += LAP_TC) {
-        const int nc = n - c0 < LAP_TC ? n - c0 : LAP_TC;
+    const int cbeg = blockIdx.y * tps * LAP_TC;
+    const int cend = min(n, cbeg + tps * LAP_TC);
+    for (int c0 = cbeg; c0 < cend; c0 += LAP_TC) {
+        const int nc = cend - c0 < LAP_TC ? cend - c0 : LAP_TC;
         __syncthreads();  // the previous tile is no longer read
         for (int e = tid; e < LAP_TC * DMAX; e += LAP_NT) {
             const int ci = e / DMAX, j = e % DMAX;
@@ -73,24 +101,40 @@ __global__ __launch_bounds__(256) void k_lap_knn(const T* __restrict__ x, const 
             T dot = 0;
 #pragma unroll
             for (int j = 0; j < DMAX; j++) dot = fma(qv[j], ct[ci][j], dot);
-            T v = fma(T(-2), dot, nq + cn[ci]);
-            int c = c0 + ci;
-            if (lap_before(v, c, lv[KMAX - 1], li[KMAX - 1])) {
-                // insertion into the sorted register list (static indices only)
-#pragma unroll
-                for (int j = 0; j < KMAX; j++) {
-                    const bool lt = lap_before(v, c, lv[j], li[j]);
-                    const T tv = lt ? lv[j] : v;
-                    const int tc = lt ? li[j] : c;
-                    lv[j] = lt ? v : lv[j];
-                    li[j] = lt ? c : li[j];
-                    v = tv;
-                    c = tc;
-                }
-            }
+            lap_offer<T, KMAX>(lv, li, fma(T(-2), dot, nq + cn[ci]), c0 + ci);
         }
     }
     if (!active) return;
+    const size_t o = ((size_t)blockIdx.y * n + q) * KMAX;
+#pragma unroll
+    for (int j = 0; j < KMAX; j++) {
+        pv[o + j] = lv[j];
+        pc[o + j] = li[j];
+    }
+}
+
+// One thread per query: the k smallest of the slices' lists, in order.
+template <typename T, int KMAX>
+__global__ __launch_bounds__(256) void k_lap_merge(const T* __restrict__ pv, const int* __restrict__ pc, int n,
+                                                   int slices, int k, int64_t* __restrict__ cols, T* __restrict__ vals) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    T lv[KMAX];
+    int li[KMAX];
+#pragma unroll
+    for (int j = 0; j < KMAX; j++) {
+        lv[j] = __builtin_inf();
+        li[j] = 0x7fffffff;
+    }
+    for (int sl = 0; sl < slices; sl++) {
+        const size_t o = ((size_t)sl * n + q) * KMAX;
+        for (int j = 0; j < KMAX; j++) {
+            const T v = pv[o + j];
+            const int c = pc[o + j];
+            if (!lap_before(v, c, lv[KMAX - 1], li[KMAX - 1])) break;  // the slice's list is sorted
+            lap_offer<T, KMAX>(lv, li, v, c);
+        }
+    }
 #pragma unroll
     for (int j = 0; j < KMAX; j++) {
         if (j < k) {
@@ -116,43 +160,60 @@ __device__ __forceinline__ double lap_unkey(uint64_t u) {
     return __longlong_as_double((long long)((u & 0x8000000000000000ull) ? (u & 0x7fffffffffffffffull) : ~u));
 }
 
-// Value of rank (m - 1) / 2 among m values (torch.median's lower median): radix
-// select, 8 bits per pass from the top, one 1024-thread block.
+// Radix-select state in device memory: histogram of the current pass, the key
+// prefix found so far and the rank still to skip within it.
+struct LapSel {
+    unsigned int hist[256];
+    unsigned long long prefix;
+    unsigned long long rank;
+};
+
+__global__ void k_lap_sel_init(LapSel* st, int64_t m) {
+    const int t = threadIdx.x;
+    st->hist[t] = 0u;
+    if (t == 0) {
+        st->prefix = 0ull;
+        st->rank = (unsigned long long)((m - 1) / 2);  // torch.median: the lower middle element
+    }
+}
+
+// Histogram of digit (key >> shift) & 255 over the values whose higher bits equal the prefix.
 template <typename T>
-__global__ __launch_bounds__(1024) void k_lap_median(const T* __restrict__ v, int64_t m, T* __restrict__ out) {
+__global__ __launch_bounds__(256) void k_lap_hist(const T* __restrict__ v, int64_t m, int shift, LapSel* st) {
     using K = typename std::conditional<sizeof(T) == 4, uint32_t, uint64_t>::type;
     constexpr int BITS = sizeof(T) * 8;
-    __shared__ unsigned int hist[256];
-    __shared__ K s_prefix;
-    __shared__ unsigned long long s_rank;
-    const int tid = threadIdx.x;
-    if (tid == 0) {
-        s_prefix = 0;
-        s_rank = (unsigned long long)((m - 1) / 2);
+    __shared__ unsigned int h[256];
+    h[threadIdx.x] = 0u;
+    __syncthreads();
+    const K prefix = (K)st->prefix;
+    const K hmask = shift + 8 >= BITS ? K(0) : (~K(0) << (shift + 8));
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x) {
+        const K key = lap_key(v[i]);
+        if ((key & hmask) == prefix) atomicAdd(&h[(key >> shift) & 255], 1u);
     }
-    for (int shift = BITS - 8; shift >= 0; shift -= 8) {
-        if (tid < 256) hist[tid] = 0;
-        __syncthreads();
-        const K prefix = s_prefix;
-        const K hmask = shift + 8 >= BITS ? K(0) : (~K(0) << (shift + 8));
-        for (int64_t i = tid; i < m; i += blockDim.x) {
-            const K key = lap_key(v[i]);
-            if ((key & hmask) == prefix) atomicAdd(&hist[(key >> shift) & 255], 1u);
+    __syncthreads();
+    if (h[threadIdx.x]) atomicAdd(&st->hist[threadIdx.x], h[threadIdx.x]);
+}
+
+// Digit of the rank-th value in this pass; clears the histogram for the next pass.
+template <typename T>
+__global__ void k_lap_pick(LapSel* st, int shift, T* __restrict__ out) {
+    if (threadIdx.x == 0) {
+        unsigned long long r = st->rank, acc = 0;
+        int dig = 0;
+        for (; dig < 255; dig++) {
+            if (acc + st->hist[dig] > r) break;
+            acc += st->hist[dig];
         }
-        __syncthreads();
-        if (tid == 0) {
-            unsigned long long r = s_rank, acc = 0;
-            int dig = 0;
-            for (; dig < 255; dig++) {
-                if (acc + hist[dig] > r) break;
-                acc += hist[dig];
-            }
-            s_rank = r - acc;
-            s_prefix = prefix | ((K)dig << shift);
+        st->rank = r - acc;
+        st->prefix |= (unsigned long long)dig << shift;
+        if (shift == 0) {
+            using K = typename std::conditional<sizeof(T) == 4, uint32_t, uint64_t>::type;
+            out[0] = lap_unkey((K)st->prefix);
         }
-        __syncthreads();
     }
-    if (tid == 0) out[0] = lap_unkey(s_prefix);
+    __syncthreads();
+    st->hist[threadIdx.x] = 0u;
 }
 
 // One thread per point: w_j = exp(-v_j / median) / sum_j exp(-v_j / median), as float32.
@@ -187,42 +248,94 @@ bool lap_shape(int d, int k, int dtype, LapShape* s) {
     return s->dmax * words + s->kmax * (words + 1) <= 192;
 }
 
+// Candidate slices: enough (query block, slice) pairs for the chip, at least one tile each.
+int lap_slices(int64_t n) {
+    const int64_t qb = (n + LAP_NT - 1) / LAP_NT;
+    const int64_t tiles = (n + LAP_TC - 1) / LAP_TC;
+    int64_t s = (LAP_GRID + qb - 1) / qb;
+    if (s > tiles) s = tiles;
+    if (s > 64) s = 64;
+    return (int)(s < 1 ? 1 : s);
+}
+
+size_t lap_align(size_t b) { return (b + 255) & ~(size_t)255; }
+
+struct LapWs {
+    size_t norm, sel, part_v, part_c, state, median, total;
+};
+
+LapWs lap_layout(int64_t n, int k, int dtype, int kmax) {
+    const size_t es = dtype == CTLIB_F64 ? 8 : 4;
+    const size_t parts = (size_t)lap_slices(n) * n * kmax;
+    LapWs L;
+    size_t off = 0;
+    L.norm = off; off += lap_align(n * es);
+    L.sel = off; off += lap_align((size_t)n * k * es);
+    L.part_v = off; off += lap_align(parts * es);
+    L.part_c = off; off += lap_align(parts * sizeof(int));
+    L.state = off; off += lap_align(sizeof(LapSel));
+    L.median = off; off += lap_align(es);
+    L.total = off;
+    return L;
+}
+
 template <typename T, int DMAX, int KMAX>
-hipError_t knn_launch(const void* x, const void* norm, int n, int d, int k, int64_t* cols, void* vals, hipStream_t s) {
-    hipLaunchKernelGGL((k_lap_knn<T, DMAX, KMAX>), dim3((n + LAP_NT - 1) / LAP_NT), dim3(LAP_NT), 0, s, (const T*)x,
-                       (const T*)norm, n, d, k, cols, (T*)vals);
+hipError_t knn_launch(const void* x, const void* norm, int n, int d, int k, int64_t* cols, char* w, const LapWs& L,
+                      hipStream_t s) {
+    const int slices = lap_slices(n);
+    const int tiles = (n + LAP_TC - 1) / LAP_TC;
+    const int tps = (tiles + slices - 1) / slices;
+    const dim3 grid((n + LAP_NT - 1) / LAP_NT, slices);
+    hipLaunchKernelGGL((k_lap_knn<T, DMAX, KMAX>), grid, dim3(LAP_NT), 0, s, (const T*)x, (const T*)norm, n, d, tps,
+                       (T*)(w + L.part_v), (int*)(w + L.part_c));
+    hipLaunchKernelGGL((k_lap_merge<T, KMAX>), dim3((n + 255) / 256), dim3(256), 0, s, (const T*)(w + L.part_v),
+                       (const int*)(w + L.part_c), n, slices, k, cols, (T*)(w + L.sel));
     return hipGetLastError();
 }
 
 template <typename T, int DMAX>
-hipError_t knn_k(const LapShape& sh, const void* x, const void* norm, int n, int d, int k, int64_t* cols, void* vals,
-                 hipStream_t s) {
+hipError_t knn_k(const LapShape& sh, const void* x, const void* norm, int n, int d, int k, int64_t* cols, char* w,
+                 const LapWs& L, hipStream_t s) {
     switch (sh.kmax) {
-    case 8: return knn_launch<T, DMAX, 8>(x, norm, n, d, k, cols, vals, s);
-    case 16: return knn_launch<T, DMAX, 16>(x, norm, n, d, k, cols, vals, s);
+    case 8: return knn_launch<T, DMAX, 8>(x, norm, n, d, k, cols, w, L, s);
+    case 16: return knn_launch<T, DMAX, 16>(x, norm, n, d, k, cols, w, L, s);
     case 32:
         if constexpr (DMAX * (sizeof(T) / 4) + 32 * (sizeof(T) / 4 + 1) <= 192)
-            return knn_launch<T, DMAX, 32>(x, norm, n, d, k, cols, vals, s);
+            return knn_launch<T, DMAX, 32>(x, norm, n, d, k, cols, w, L, s);
         return hipErrorInvalidValue;
     default:
         if constexpr (DMAX * (sizeof(T) / 4) + 64 * (sizeof(T) / 4 + 1) <= 192)
-            return knn_launch<T, DMAX, 64>(x, norm, n, d, k, cols, vals, s);
+            return knn_launch<T, DMAX, 64>(x, norm, n, d, k, cols, w, L, s);
         return hipErrorInvalidValue;
     }
 }
 
 template <typename T>
-hipError_t knn(const LapShape& sh, const void* x, const void* norm, int n, int d, int k, int64_t* cols, void* vals,
-               hipStream_t s) {
+hipError_t run(const LapShape& sh, const void* x, int64_t n, int d, int k, int64_t* cols, float* vals, char* w,
+               const LapWs& L, hipStream_t s) {
+    const dim3 g1((unsigned)((n + 255) / 256));
+    hipLaunchKernelGGL(k_lap_norm<T>, g1, dim3(256), 0, s, (const T*)x, n, d, (T*)(w + L.norm));
+    hipError_t e;
     switch (sh.dmax) {
-    case 16: return knn_k<T, 16>(sh, x, norm, n, d, k, cols, vals, s);
-    case 32: return knn_k<T, 32>(sh, x, norm, n, d, k, cols, vals, s);
-    case 64: return knn_k<T, 64>(sh, x, norm, n, d, k, cols, vals, s);
-    default: return knn_k<T, 128>(sh, x, norm, n, d, k, cols, vals, s);
+    case 16: e = knn_k<T, 16>(sh, x, w + L.norm, (int)n, d, k, cols, w, L, s); break;
+    case 32: e = knn_k<T, 32>(sh, x, w + L.norm, (int)n, d, k, cols, w, L, s); break;
+    case 64: e = knn_k<T, 64>(sh, x, w + L.norm, (int)n, d, k, cols, w, L, s); break;
+    default: e = knn_k<T, 128>(sh, x, w + L.norm, (int)n, d, k, cols, w, L, s); break;
     }
+    if (e != hipSuccess) return e;
+    const int64_t m = n * k;
+    LapSel* st = reinterpret_cast<LapSel*>(w + L.state);
+    T* med = reinterpret_cast<T*>(w + L.median);
+    const T* sel = reinterpret_cast<const T*>(w + L.sel);
+    hipLaunchKernelGGL(k_lap_sel_init, dim3(1), dim3(256), 0, s, st, m);
+    const int hb = (int)std::min<int64_t>(1024, (m + 255) / 256);
+    for (int shift = (int)sizeof(T) * 8 - 8; shift >= 0; shift -= 8) {
+        hipLaunchKernelGGL(k_lap_hist<T>, dim3(hb), dim3(256), 0, s, sel, m, shift, st);
+        hipLaunchKernelGGL(k_lap_pick<T>, dim3(1), dim3(256), 0, s, st, shift, med);
+    }
+    hipLaunchKernelGGL(k_lap_weights<T>, g1, dim3(256), 0, s, sel, (const T*)med, n, k, vals);
+    return hipGetLastError();
 }
-
-size_t lap_align(size_t b) { return (b + 255) & ~(size_t)255; }
 
 }  // namespace
 
@@ -231,37 +344,20 @@ bool laplacian_supported(int d, int k, int dtype) {
     return lap_shape(d, k, dtype, &sh);
 }
 
-size_t laplacian_ws_bytes(int64_t n, int k, int dtype) {
-    const size_t es = dtype == CTLIB_F64 ? 8 : 4;
-    return lap_align(n * es) + lap_align((size_t)n * k * es) + lap_align(es);
+size_t laplacian_ws_bytes(int64_t n, int d, int k, int dtype) {
+    LapShape sh;
+    if (!lap_shape(d, k, dtype, &sh)) return 0;
+    return lap_layout(n, k, dtype, sh.kmax).total;
 }
 
 hipError_t launch_laplacian(const void* x, int64_t n, int d, int k, int dtype, int64_t* cols, float* vals, void* ws,
                             hipStream_t s) {
     LapShape sh;
     if (!lap_shape(d, k, dtype, &sh)) return hipErrorInvalidValue;
-    const size_t es = dtype == CTLIB_F64 ? 8 : 4;
+    const LapWs L = lap_layout(n, k, dtype, sh.kmax);
     char* w = static_cast<char*>(ws);
-    void* norm = w;
-    void* sel = w + lap_align(n * es);
-    void* med = w + lap_align(n * es) + lap_align((size_t)n * k * es);
-    const dim3 g1((unsigned)((n + 255) / 256));
-    hipError_t e;
-    if (dtype == CTLIB_F32) {
-        hipLaunchKernelGGL(k_lap_norm<float>, g1, dim3(256), 0, s, (const float*)x, n, d, (float*)norm);
-        e = knn<float>(sh, x, norm, (int)n, d, k, cols, sel, s);
-        if (e != hipSuccess) return e;
-        hipLaunchKernelGGL(k_lap_median<float>, dim3(1), dim3(1024), 0, s, (const float*)sel, n * k, (float*)med);
-        hipLaunchKernelGGL(k_lap_weights<float>, g1, dim3(256), 0, s, (const float*)sel, (const float*)med, n, k, vals);
-    } else {
-        hipLaunchKernelGGL(k_lap_norm<double>, g1, dim3(256), 0, s, (const double*)x, n, d, (double*)norm);
-        e = knn<double>(sh, x, norm, (int)n, d, k, cols, sel, s);
-        if (e != hipSuccess) return e;
-        hipLaunchKernelGGL(k_lap_median<double>, dim3(1), dim3(1024), 0, s, (const double*)sel, n * k, (double*)med);
-        hipLaunchKernelGGL(k_lap_weights<double>, g1, dim3(256), 0, s, (const double*)sel, (const double*)med, n, k,
-                           vals);
-    }
-    return hipGetLastError();
+    if (dtype == CTLIB_F32) return run<float>(sh, x, n, d, k, cols, vals, w, L, s);
+    return run<double>(sh, x, n, d, k, cols, vals, w, L, s);
 }
 
 }  // namespace ctlib
