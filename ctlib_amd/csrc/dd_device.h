@@ -651,18 +651,14 @@ __device__ __forceinline__ void adj_block(const DevPlan& P, const double2* __res
                         double right = shfl_next_d(gprev);
                         const double r0 = readlane_d(gi, 0);
                         if (lane == 63) right = r0;
-                        const int p = lane + 64 * (i - 1);
-                        acc[q][i - 1] += p < n ? wgt(p) * (right - gprev) : 0.0;
+                        // pixels past the line accumulate unused values (never stored)
+                        acc[q][i - 1] += wgt(lane + 64 * (i - 1)) * (right - gprev);
                     }
                     gprev = gi;
                 }
                 // pixels of the last slot group (their right edges are in the same group;
                 // lane 63's pixel is past the line since 64 * NPE > n)
-                {
-                    const double right = shfl_next_d(gprev);
-                    const int p = lane + 64 * (NPE - 1);
-                    acc[q][NPE - 1] += p < n ? wgt(p) * (right - gprev) : 0.0;
-                }
+                acc[q][NPE - 1] += wgt(lane + 64 * (NPE - 1)) * (shfl_next_d(gprev) - gprev);
             }
         }
         if (iv + 1 < nv && cthread) scf[(cur ^ 1) * L * LC_WORDS + tid] = cnext;
