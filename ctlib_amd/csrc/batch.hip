@@ -105,7 +105,7 @@ __global__ void k_batch_reset(unsigned int* ctr, int n) {
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_batch(const SliceDev* __restrict__ slices, const Item* __restrict__ items,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_batch(const SliceDev* __restrict__ slices, const Item* __restrict__ items,
                                                int nitems, unsigned int* __restrict__ ctr, int nslices) {
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ int s_item;
