@@ -81,7 +81,8 @@ def build(arch="gfx950", verbose=False, with_torch=True):
     with ThreadPoolExecutor(max_workers=min(len(jobs), os.cpu_count() or 4)) as ex:
         list(ex.map(lambda c: _run(c, verbose), jobs))
     _run([HIPCC, f"--offload-arch={arch}", "-shared", "-fPIC", "-o", LIB + ".tmp",
-          *[os.path.join(BUILD, n) for n in objs]], verbose)
+          *[os.path.join(BUILD, n) for n in objs], f"-L{ROCM}/lib", "-lrocprofiler-sdk-roctx",
+          f"-Wl,-rpath,{ROCM}/lib"], verbose)
     shutil.move(LIB + ".tmp", LIB)
     if ext_cmd:
         _run(ext_cmd, verbose)

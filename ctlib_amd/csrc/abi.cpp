@@ -11,6 +11,7 @@
 //   * invalid input returns a status code (the reference exit(0)s on an unknown
 //     scan_type, ctlib.cpp:26).
 #include <hip/hip_runtime.h>
+#include <rocprofiler-sdk-roctx/roctx.h>
 
 #include <cstring>
 #include <mutex>
@@ -48,6 +49,20 @@ int fail(int code, const std::string& why) {
 int hip_fail(hipError_t e, const char* what) {
     g_err = std::string(what) + ": " + hipGetErrorString(e);
     return CTLIB_ERR_HIP;
+}
+
+// roctx range around one C-ABI call, so that a marker trace (rocprofv3
+// --marker-trace) groups each call's kernels under the operator's name.
+struct TraceRange {
+    explicit TraceRange(const char* name) { roctxRangePushA(name); }
+    ~TraceRange() { roctxRangePop(); }
+};
+
+const char* op_range_name(int op) {
+    static const char* names[] = {"ctlib:projection", "ctlib:projection_t", "ctlib:backprojection",
+                                  "ctlib:backprojection_t", "ctlib:backprojection_t_true", "ctlib:fbp",
+                                  "ctlib:fbp_t", "ctlib:backprojection_sv"};
+    return op >= 0 && op < (int)(sizeof(names) / sizeof(names[0])) ? names[op] : "ctlib:op";
 }
 
 // Restores the calling thread's current device on scope exit.
@@ -479,6 +494,7 @@ int ctlib_run(const ctlib_plan* p, int op, int dtype, const void* in, void* out,
     if (batch == 0) return CTLIB_OK;
     DeviceGuard guard(p->device);
     if (!guard.ok) return fail(CTLIB_ERR_HIP, "cannot select the plan's device");
+    TraceRange range(op_range_name(op));
     const DevPlan& P = p->dev;
     const int B = (int)batch;
     const int dt = dtype == CTLIB_F64 ? DT_F64 : DT_F32;
@@ -553,6 +569,7 @@ int ctlib_laplacian(const void* x, int64_t n, int d, int k, int dtype, int64_t* 
                                            "2 dimension + 3k <= 192 (fp64)");
     if (!ws || ws_bytes < ctlib_laplacian_workspace_bytes(n, d, k, dtype))
         return fail(CTLIB_ERR_WORKSPACE, "workspace smaller than ctlib_laplacian_workspace_bytes()");
+    TraceRange range("ctlib:laplacian");
     hipError_t e = launch_laplacian(x, n, d, k, dtype, cols, vals, ws, static_cast<hipStream_t>(stream));
     if (e != hipSuccess) return hip_fail(e, "laplacian launch");
     return CTLIB_OK;
@@ -606,6 +623,7 @@ int ctlib_run_residual(const ctlib_plan* p, int dtype, const void* x, const void
         return fail(CTLIB_ERR_WORKSPACE, "workspace smaller than ctlib_residual_workspace_bytes()");
     DeviceGuard guard(p->device);
     if (!guard.ok) return fail(CTLIB_ERR_HIP, "cannot select the plan's device");
+    TraceRange range("ctlib:residual_step");
     const DevPlan& P = p->dev;
     const int B = (int)batch;
     const int dt = dtype == CTLIB_F64 ? DT_F64 : DT_F32;

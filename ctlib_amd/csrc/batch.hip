@@ -16,6 +16,7 @@
 // records no memset (counters are reset by a kernel), so it can be captured in
 // a HIP graph and replayed.
 #include <hip/hip_runtime.h>
+#include <rocprofiler-sdk-roctx/roctx.h>
 
 #include <algorithm>
 #include <cstring>
@@ -389,6 +390,7 @@ int ctlib_batch_run(const ctlib_batch* b, void* stream) {
     int prev = -1;
     (void)hipGetDevice(&prev);
     (void)hipSetDevice(b->device);
+    roctxRangePushA("ctlib:batch_residual_step");
     hipStream_t s = static_cast<hipStream_t>(stream);
     hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(256), 0, s, b->d_ctr, b->nctr);
     hipError_t e = hipGetLastError();
@@ -408,6 +410,7 @@ int ctlib_batch_run(const ctlib_batch* b, void* stream) {
             e = hipGetLastError();
         }
     }
+    roctxRangePop();
     if (prev >= 0) (void)hipSetDevice(prev);
     if (e != hipSuccess) return bfail(CTLIB_ERR_HIP, std::string("batch launch: ") + hipGetErrorString(e));
     return CTLIB_OK;
