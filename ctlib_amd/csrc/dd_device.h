@@ -408,10 +408,13 @@ __host__ __device__ inline size_t fwd_w_block_lds(int nmax) { return (size_t)4 *
 // by itself: lane L owns pixels L*PPL .. L*PPL+PPL-1 (one serial sum and one
 // wave scan per line) and writes them into its wave's private table, so no
 // barrier is needed anywhere.  The line's pixels and the view's LineCoef words
-// are loaded one line ahead into registers.  Same arithmetic as fwd_block with
-// WEIGHTED (pixel weight with a fp32 reciprocal, weighted value rounded to the
-// storage type before the prefix sum).
-template <typename T, int NEM, int PPL>
+// are loaded one line ahead into registers.  FULL: every line has exactly
+// 64*PPL pixels, so no lane needs a bounds predicate.  The pixel weight
+// 1/(s ((w0 + w1 p)^2 + w2)) is formed in the data precision (fp32 data: fp32
+// arithmetic with a fp32 reciprocal, about 1e-7 relative; fp64 data: fp64
+// arithmetic with a fp32 reciprocal), and the weighted value is rounded to the
+// data type before the fp64 prefix sum, as in fwd_block with WEIGHTED.
+template <typename T, int NEM, int PPL, bool FULL>
 __device__ __forceinline__ void fwd_block_w(const DevPlan& P, const T* __restrict__ src, T* __restrict__ y, int b,
                                             int mode, int grp, const double* __restrict__ post,
                                             unsigned char* __restrict__ smem) {
@@ -420,7 +423,7 @@ __device__ __forceinline__ void fwd_block_w(const DevPlan& P, const T* __restric
     const int* vlist = mode == MODE_Y ? P.yviews : P.xviews;
     const int nv = mode == MODE_Y ? P.ny : P.nx;
     const int nlines = mode == MODE_Y ? P.W : P.H;
-    const int n = mode == MODE_Y ? P.H : P.W;
+    const int n = FULL ? 64 * PPL : (mode == MODE_Y ? P.H : P.W);
     const int D = P.D;
 
     int vw[1];
@@ -445,7 +448,7 @@ __device__ __forceinline__ void fwd_block_w(const DevPlan& P, const T* __restric
     auto load_line = [&](int l, T (&xv)[PPL], double& cw) {
         const T* row = base + (size_t)l * n;
 #pragma unroll
-        for (int k = 0; k < PPL; k++) xv[k] = p0 + k < n ? row[p0 + k] : T(0);
+        for (int k = 0; k < PPL; k++) xv[k] = (FULL || p0 + k < n) ? row[p0 + k] : T(0);
         cw = crow[(size_t)l * LC_WORDS];
     };
     T xv[PPL];
@@ -456,14 +459,28 @@ __device__ __forceinline__ void fwd_block_w(const DevPlan& P, const T* __restric
         const double s = readlane_d(cw, LC_S), A = readlane_d(cw, LC_A), Bc = readlane_d(cw, LC_B);
         const double w0 = readlane_d(cw, LC_W0), w1 = readlane_d(cw, LC_W1);
         const double w2 = readlane_d(cw, LC_W2), is = readlane_d(cw, LC_INVS);
-        double val[PPL], run = 0.0;
+        double val[PPL];
+        if constexpr (sizeof(T) == 4) {
+            const float w0f = (float)w0, w1f = (float)w1, w2f = (float)w2, isf = (float)is;
 #pragma unroll
-        for (int k = 0; k < PPL; k++) {
-            const double u = __builtin_fma(w1, (double)(p0 + k), w0);
-            const double d = __builtin_fma(u, u, w2);
-            val[k] = p0 + k < n ? (double)(T)((double)xv[k] * (is * (double)__builtin_amdgcn_rcpf((float)d))) : 0.0;
-            run += val[k];
+            for (int k = 0; k < PPL; k++) {
+                const float u = __builtin_fmaf(w1f, (float)(p0 + k), w0f);
+                const float d = __builtin_fmaf(u, u, w2f);
+                const float v = (float)xv[k] * (isf * __builtin_amdgcn_rcpf(d));
+                val[k] = (FULL || p0 + k < n) ? (double)v : 0.0;
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < PPL; k++) {
+                const double u = __builtin_fma(w1, (double)(p0 + k), w0);
+                const double d = __builtin_fma(u, u, w2);
+                const double v = (double)xv[k] * (is * (double)__builtin_amdgcn_rcpf((float)d));
+                val[k] = (FULL || p0 + k < n) ? v : 0.0;
+            }
         }
+        double run = 0.0;
+#pragma unroll
+        for (int k = 0; k < PPL; k++) run += val[k];
         if (l + 1 < nlines) load_line(l + 1, xv, cw);
         // Table of this line: the wave's reads of the previous line's table were
         // consumed above (in-order LDS within a wave), so it can be overwritten.
@@ -472,7 +489,7 @@ __device__ __forceinline__ void fwd_block_w(const DevPlan& P, const T* __restric
 #pragma unroll
         for (int k = 0; k < PPL; k++) {
             const int p = p0 + k;
-            if (p < n) tb[wtab_slot(p + 1)] = make_double2(__builtin_fma(-(double)(p + 1), val[k], pref), val[k]);
+            if (FULL || p < n) tb[wtab_slot(p + 1)] = make_double2(__builtin_fma(-(double)(p + 1), val[k], pref), val[k]);
             pref += val[k];
         }
         if (lane == 0) tb[0] = make_double2(0.0, 0.0);

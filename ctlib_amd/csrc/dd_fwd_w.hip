@@ -14,8 +14,13 @@ namespace {
 
 // Work items [0, B*ngx) are (b, group of 4 X-axis views) reading image rows from
 // `srcx`; the rest are Y-axis groups reading the bottom-to-top column copy.
-template <typename T, int NEM, int PPL>
-__global__ __launch_bounds__(256) void k_fwd_w(DevPlan P, const T* __restrict__ srcx, const T* __restrict__ srcy,
+// fp32 variants for 512-pixel lines are held to 128 VGPRs: four waves per
+// SIMD, as many as the LDS of four 512-pixel tables per block allows.
+template <typename T, int NEM, int PPL, bool FULL>
+constexpr int fwd_w_waves() { return sizeof(T) == 4 && FULL && PPL == 8 && NEM <= 12 ? 4 : 1; }
+
+template <typename T, int NEM, int PPL, bool FULL>
+__global__ __launch_bounds__(256, (fwd_w_waves<T, NEM, PPL, FULL>())) void k_fwd_w(DevPlan P, const T* __restrict__ srcx, const T* __restrict__ srcy,
                                                T* __restrict__ y, int B, int ngx, int ngy,
                                                const double* __restrict__ post) {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -26,7 +31,7 @@ __global__ __launch_bounds__(256) void k_fwd_w(DevPlan P, const T* __restrict__ 
     if (mode == MODE_Y) w -= nwx;
     const int ngroups = mode == MODE_Y ? ngy : ngx;
     const int b = w / ngroups, grp = w - b * ngroups;
-    fwd_block_w<T, NEM, PPL>(P, mode == MODE_Y ? srcy : srcx, y, b, mode, grp, post, smem);
+    fwd_block_w<T, NEM, PPL, FULL>(P, mode == MODE_Y ? srcy : srcx, y, b, mode, grp, post, smem);
 }
 
 int line_max(const DevPlan& P) {
@@ -43,12 +48,12 @@ struct Args {
     const double* post;
 };
 
-template <typename T, int NEM, int PPL>
+template <typename T, int NEM, int PPL, bool FULL>
 hipError_t launch_t(const DevPlan& P, const Args& a, hipStream_t s) {
     const int ngx = (P.nx + 3) / 4, ngy = (P.ny + 3) / 4;
     const int nwork = (ngx + ngy) * a.B;
     const size_t lds = fwd_w_block_lds(line_max(P));
-    auto kern = k_fwd_w<T, NEM, PPL>;
+    auto kern = k_fwd_w<T, NEM, PPL, FULL>;
     if (lds > 64 * 1024) {
         hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
@@ -58,13 +63,22 @@ hipError_t launch_t(const DevPlan& P, const Args& a, hipStream_t s) {
     return hipGetLastError();
 }
 
+// Every swept line has exactly n pixels (rows of the X-axis views, columns of the Y-axis views).
+bool all_lines(const DevPlan& P, int n) { return (P.nx == 0 || P.W == n) && (P.ny == 0 || P.H == n); }
+
+template <typename T, int NEM, int PPL>
+hipError_t by_full(const DevPlan& P, const Args& a, hipStream_t s) {
+    if (all_lines(P, 64 * PPL)) return launch_t<T, NEM, PPL, true>(P, a, s);
+    return launch_t<T, NEM, PPL, false>(P, a, s);
+}
+
 // Pixels per lane: 64 * PPL >= n.
 template <typename T, int NEM>
 hipError_t by_ppl(const DevPlan& P, const Args& a, hipStream_t s) {
     const int n = line_max(P);
-    if (n <= 256) return launch_t<T, NEM, 4>(P, a, s);
-    if (n <= 512) return launch_t<T, NEM, 8>(P, a, s);
-    if (n <= 1024) return launch_t<T, NEM, 16>(P, a, s);
+    if (n <= 256) return by_full<T, NEM, 4>(P, a, s);
+    if (n <= 512) return by_full<T, NEM, 8>(P, a, s);
+    if (n <= 1024) return by_full<T, NEM, 16>(P, a, s);
     return hipErrorInvalidValue;
 }
 
