@@ -490,7 +490,10 @@ def main():
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
             "kernel": "k_fwd (line->bin)" if k == "fwd" else "k_adj (bin->pixel)",
-            "kernel_ms": round(per_launch_ms, 4), "algo_bytes_per_launch": per_launch_bytes}
+            "kernel_ms": round(per_launch_ms, 4), "algo_bytes_per_launch": per_launch_bytes,
+            # compulsory bytes of one operator application: the image and the sinogram once each
+            "compulsory_bytes_per_launch": 4 * B * (512 * 512 + 736 * 736),
+            "measured_hbm_gbs": round(traffic / (per_launch_ms * 1e-3) / 1e9, 1) if traffic else None}
     kernels = {c: {"launches": stats[c]["launches"],
                    "ms_per_launch": round(stats[c]["ms"] / max(stats[c]["launches"], 1), 4)}
                for c in ("fwd", "adj", "ramp", "aux")}
