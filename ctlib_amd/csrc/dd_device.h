@@ -87,6 +87,9 @@ __device__ __forceinline__ double dpp_d(double v) {
 // Lane i receives lane i+1's value (DPP wave_shl:1, no LDS round trip); lane 63 receives 0.
 __device__ __forceinline__ double shfl_next_d(double v) { return dpp_d<0x130, 0xf>(v); }
 
+// Lane i receives lane (i+1) mod 64's value (DPP wave_rol:1).
+__device__ __forceinline__ double rot_next_d(double v) { return dpp_d<0x134, 0xf>(v); }
+
 // Inclusive wave64 scan without LDS round trips: row_shr 1, 2, 4, 8 scan each
 // 16-lane row, then row_bcast:15 adds row 0's total to row 1 and row 2's to
 // row 3, and row_bcast:31 adds lane 31's running total to rows 2 and 3.
@@ -660,22 +663,23 @@ __device__ __forceinline__ void adj_block(const DevPlan& P, const double2* __res
                     const double d = __builtin_fma(u, u, w2);
                     return ws * (double)__builtin_amdgcn_rcpf((float)d);
                 };
-                double gprev = 0.0;
+                // right edge of pixel lane + 64 (i-1): edge lane+1 of group i-1, or for
+                // lane 63 edge 0 of group i; both come from the rotations of the groups
+                double gprev = 0.0, rprev = 0.0;
 #pragma unroll
                 for (int i = 0; i < NPE; i++) {
                     const double gi = eval_G<GEO>(t, a, s, lane + 64 * i, lu, hi);
+                    const double ri = rot_next_d(gi);
                     if (i > 0) {
-                        double right = shfl_next_d(gprev);
-                        const double r0 = readlane_d(gi, 0);
-                        if (lane == 63) right = r0;
+                        const double right = lane == 63 ? ri : rprev;
                         // pixels past the line accumulate unused values (never stored)
                         acc[q][i - 1] += wgt(lane + 64 * (i - 1)) * (right - gprev);
                     }
                     gprev = gi;
+                    rprev = ri;
                 }
-                // pixels of the last slot group (their right edges are in the same group;
-                // lane 63's pixel is past the line since 64 * NPE > n)
-                acc[q][NPE - 1] += wgt(lane + 64 * (NPE - 1)) * (shfl_next_d(gprev) - gprev);
+                // pixels of the last slot group (lane 63's pixel is past the line since 64 * NPE > n)
+                acc[q][NPE - 1] += wgt(lane + 64 * (NPE - 1)) * (rprev - gprev);
             }
         }
         if (iv + 1 < nv && cthread) scf[(cur ^ 1) * L * LC_WORDS + tid] = cnext;
