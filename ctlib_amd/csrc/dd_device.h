@@ -234,8 +234,9 @@ __device__ __forceinline__ void fwd_store(const DevPlan& P, const double (&S)[VP
 // Builds NTAB line tables {C_j, slope_j}, j = 0..n+1, table t from val[t] (this
 // thread's pixels p0 .. p0+m-1; already rounded to the storage precision of
 // the slope).  Table t starts at tb + t*tabn; `ws` holds NTAB x 4 wave sums.
-// One barrier inside, whatever NTAB.
-template <typename T, int M, int NTAB>
+// One barrier inside, whatever NTAB.  FULL: every thread owns exactly M pixels
+// (n == 256*M), so no pixel predicate is needed.
+template <typename T, int M, int NTAB, bool FULL = false>
 __device__ __forceinline__ void line_tables(const double (&val)[NTAB][M], int m, int p0, int n, LEnt<T>* __restrict__ tb,
                                             int tabn, double* __restrict__ ws, int tid, int lane, int wave) {
     double run[NTAB], inc[NTAB];
@@ -244,7 +245,7 @@ __device__ __forceinline__ void line_tables(const double (&val)[NTAB][M], int m,
         run[t] = 0;
 #pragma unroll
         for (int i = 0; i < M; i++)
-            if (i < m) run[t] += val[t][i];
+            if (FULL || i < m) run[t] += val[t][i];
         inc[t] = wave_scan_incl(run[t], lane);
         if (lane == 63) ws[t * 4 + wave] = inc[t];
     }
@@ -252,19 +253,20 @@ __device__ __forceinline__ void line_tables(const double (&val)[NTAB][M], int m,
 #pragma unroll
     for (int t = 0; t < NTAB; t++) {
         LEnt<T>* tt = tb + t * tabn;
-        double base = 0;
-        for (int q = 0; q < wave; q++) base += ws[t * 4 + q];
+        // the four wave totals, summed over the waves below this one (wave is uniform)
+        const double w0 = ws[t * 4], w1 = ws[t * 4 + 1], w2 = ws[t * 4 + 2], w3 = ws[t * 4 + 3];
+        const double base = (wave > 0 ? w0 : 0.0) + (wave > 1 ? w1 : 0.0) + (wave > 2 ? w2 : 0.0);
         double pref = base + inc[t] - run[t];  // sum of all pixels before p0
 #pragma unroll
         for (int i = 0; i < M; i++) {
-            if (i < m) {
+            if (FULL || i < m) {
                 const int p = p0 + i;
-                if (p < n) lent_write(tt + p + 1, pref - (double)(p + 1) * val[t][i], val[t][i]);
+                if (FULL || p < n) lent_write(tt + p + 1, pref - (double)(p + 1) * val[t][i], val[t][i]);
                 pref += val[t][i];
             }
         }
         if (tid == 0) lent_write(tt, 0.0, 0.0);
-        if (tid == NT - 1) lent_write(tt + n + 1, ws[t * 4] + ws[t * 4 + 1] + ws[t * 4 + 2] + ws[t * 4 + 3], 0.0);
+        if (tid == NT - 1) lent_write(tt + n + 1, w0 + w1 + w2 + w3, 0.0);
     }
 }
 
@@ -284,7 +286,8 @@ __host__ __device__ inline size_t fwd_block_lds(int nmax, int G, bool weighted, 
 // LDS next to it; each wave then reads only its own views' coefficients.
 // The bin values y_k = post_k (S_{k+1} - S_k) are formed with wave shuffles.
 // RESID: also write the residual bin tables of projection_t (g = (y - m) post).
-template <typename T, int VPW, int NEM, int M, bool WEIGHTED, bool RESID>
+// FULL: every line of both axes has exactly 256*M pixels (no pixel predicates).
+template <typename T, int VPW, int NEM, int M, bool WEIGHTED, bool RESID, bool FULL = false>
 __device__ __forceinline__ void fwd_block(const DevPlan& P, const T* __restrict__ src, T* __restrict__ y, int b,
                                           int mode, int grp, const double* __restrict__ post,
                                           const T* __restrict__ m, double2* __restrict__ tabs,
@@ -296,7 +299,7 @@ __device__ __forceinline__ void fwd_block(const DevPlan& P, const T* __restrict_
     const int* vlist = mode == MODE_Y ? P.yviews : P.xviews;
     const int nv = mode == MODE_Y ? P.ny : P.nx;
     const int nlines = mode == MODE_Y ? P.W : P.H;
-    const int n = mode == MODE_Y ? P.H : P.W;
+    const int n = FULL ? NT * M : (mode == MODE_Y ? P.H : P.W);
     const int D = P.D;
 
     int vw[VPW];
@@ -321,7 +324,7 @@ __device__ __forceinline__ void fwd_block(const DevPlan& P, const T* __restrict_
     double* cf = reinterpret_cast<double*>(tab + 2 * NTAB * tabn);  // [2][G][LC_WORDS]
     double* wsum = cf + 2 * G * LC_WORDS;                           // [2][NTAB][4]
     const T* base = src + (size_t)b * nlines * n;
-    const int m_px = (n + NT - 1) / NT;
+    const int m_px = FULL ? M : (n + NT - 1) / NT;
     const int p0 = tid * m_px;
     const int hi = n + 1;
 
@@ -337,7 +340,7 @@ __device__ __forceinline__ void fwd_block(const DevPlan& P, const T* __restrict_
 
     T xv[M];
 #pragma unroll
-    for (int i = 0; i < M; i++) xv[i] = (i < m_px && p0 + i < n) ? base[p0 + i] : T(0);
+    for (int i = 0; i < M; i++) xv[i] = (FULL || (i < m_px && p0 + i < n)) ? base[p0 + i] : T(0);
 
     for (int l = 0; l < nlines; l++) {
         LEnt<T>* tb = tab + (l & 1) * NTAB * tabn;
@@ -354,7 +357,7 @@ __device__ __forceinline__ void fwd_block(const DevPlan& P, const T* __restrict_
 #pragma unroll
                 for (int i = 0; i < M; i++) {
                     val[t][i] = 0.0;
-                    if (i < m_px && p0 + i < n) {
+                    if (FULL || (i < m_px && p0 + i < n)) {
                         const double u = __builtin_fma(w1, (double)(p0 + i), w0);
                         const double d = __builtin_fma(u, u, w2);
                         // rounded to the slope's storage precision before the prefix sum
@@ -363,14 +366,14 @@ __device__ __forceinline__ void fwd_block(const DevPlan& P, const T* __restrict_
                 }
             } else {
 #pragma unroll
-                for (int i = 0; i < M; i++) val[t][i] = (i < m_px && p0 + i < n) ? (double)xv[i] : 0.0;
+                for (int i = 0; i < M; i++) val[t][i] = (FULL || (i < m_px && p0 + i < n)) ? (double)xv[i] : 0.0;
             }
         }
-        line_tables<T, M, NTAB>(val, m_px, p0, n, tb, tabn, ws, tid, lane, wave);
+        line_tables<T, M, NTAB, FULL>(val, m_px, p0, n, tb, tabn, ws, tid, lane, wave);
         if (l + 1 < nlines) {
             const T* nb = base + (size_t)(l + 1) * n;
 #pragma unroll
-            for (int i = 0; i < M; i++) xv[i] = (i < m_px && p0 + i < n) ? nb[p0 + i] : T(0);
+            for (int i = 0; i < M; i++) xv[i] = (FULL || (i < m_px && p0 + i < n)) ? nb[p0 + i] : T(0);
             if (cthread) cnext = crow[(size_t)(l + 1) * LC_WORDS];
         }
         __syncthreads();

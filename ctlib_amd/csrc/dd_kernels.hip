@@ -24,8 +24,14 @@ __global__ __launch_bounds__(256) void k_to_ylines(const T* __restrict__ x, T* _
 // One launch covers both sweep axes: work items [0, B*ngx) are (b, group of
 // 4*VPW X-axis views) and read image rows from `srcx`; the rest are Y-axis
 // groups reading the bottom-to-top column copy `srcy`.
-template <typename T, int VPW, int NEM, int M, bool WEIGHTED, bool RESID>
-__global__ __launch_bounds__(256) void k_fwd(DevPlan P, const T* __restrict__ srcx, const T* __restrict__ srcy,
+// fp32 variants with 24 edge slots per lane and up to 4 pixels per thread
+// (e.g. 1536 bins over 1024-pixel lines) are held to 168 VGPRs: three waves per
+// SIMD instead of two, without spills.
+template <typename T, int NEM, int M, bool WEIGHTED>
+constexpr int fwd_waves() { return sizeof(T) == 4 && !WEIGHTED && NEM > 20 && NEM <= 24 && M <= 4 ? 3 : 1; }
+
+template <typename T, int VPW, int NEM, int M, bool WEIGHTED, bool RESID, bool FULL>
+__global__ __launch_bounds__(256, (fwd_waves<T, NEM, M, WEIGHTED>())) void k_fwd(DevPlan P, const T* __restrict__ srcx, const T* __restrict__ srcy,
                                              T* __restrict__ y, int B, int ngx, int ngy,
                                              const double* __restrict__ post, const T* __restrict__ m,
                                              double2* __restrict__ tabs) {
@@ -37,7 +43,8 @@ __global__ __launch_bounds__(256) void k_fwd(DevPlan P, const T* __restrict__ sr
     if (mode == MODE_Y) w -= nwx;
     const int ngroups = mode == MODE_Y ? ngy : ngx;
     const int b = w / ngroups, grp = w - b * ngroups;
-    fwd_block<T, VPW, NEM, M, WEIGHTED, RESID>(P, mode == MODE_Y ? srcy : srcx, y, b, mode, grp, post, m, tabs, smem);
+    fwd_block<T, VPW, NEM, M, WEIGHTED, RESID, FULL>(P, mode == MODE_Y ? srcy : srcx, y, b, mode, grp, post, m, tabs,
+                                                     smem);
 }
 
 // Per-(b, view) bin tables {C_j, g_j} of g_k = ((y - m)[o(k)] * pre[k]) * scale, j = 0..D+1.
@@ -222,13 +229,13 @@ struct FwdArgs {
     double2* tabs;
 };
 
-template <typename T, int VPW, int NEM, int M, bool WTD, bool RESID>
+template <typename T, int VPW, int NEM, int M, bool WTD, bool RESID, bool FULL>
 hipError_t fwd_launch_t(const DevPlan& P, const FwdArgs& a, hipStream_t s) {
     constexpr int G = 4 * VPW;
     const int ngx = (P.nx + G - 1) / G, ngy = (P.ny + G - 1) / G;
     const int nwork = (ngx + ngy) * a.B;
     const size_t lds = fwd_lds_bytes(P, WTD, G);
-    auto kern = k_fwd<T, VPW, NEM, M, WTD, RESID>;
+    auto kern = k_fwd<T, VPW, NEM, M, WTD, RESID, FULL>;
     if (lds > 64 * 1024) {
         hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
@@ -244,9 +251,13 @@ hipError_t fwd_m(const DevPlan& P, const FwdArgs& a, hipStream_t s) {
     int n = P.nx ? P.W : 0;
     if (P.ny && P.H > n) n = P.H;
     const int m = (n + NT - 1) / NT;
-    if (m <= 2) return fwd_launch_t<T, VPW, NEM, 2, WTD, RESID>(P, a, s);
-    if (m <= 4) return fwd_launch_t<T, VPW, NEM, 4, WTD, RESID>(P, a, s);
-    if (m <= 8) return fwd_launch_t<T, VPW, NEM, 8, WTD, RESID>(P, a, s);
+    // every swept line (rows of X views, columns of Y views) exactly NT * M pixels
+    auto full = [&](int M) { return (P.nx == 0 || P.W == NT * M) && (P.ny == 0 || P.H == NT * M); };
+    if (m <= 2) return full(2) ? fwd_launch_t<T, VPW, NEM, 2, WTD, RESID, true>(P, a, s)
+                               : fwd_launch_t<T, VPW, NEM, 2, WTD, RESID, false>(P, a, s);
+    if (m <= 4) return full(4) ? fwd_launch_t<T, VPW, NEM, 4, WTD, RESID, true>(P, a, s)
+                               : fwd_launch_t<T, VPW, NEM, 4, WTD, RESID, false>(P, a, s);
+    if (m <= 8) return fwd_launch_t<T, VPW, NEM, 8, WTD, RESID, false>(P, a, s);
     return hipErrorInvalidValue;
 }
 

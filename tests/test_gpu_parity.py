@@ -122,6 +122,26 @@ def test_full_fan_geometry(dev, cfg, op):
 
 
 @pytest.mark.parametrize("st", [0, 1, 2])
+@pytest.mark.parametrize("wh", [(256, 128), (128, 256)])
+@pytest.mark.parametrize("op", ["projection", "backprojection", "backprojection_t", "backprojection_t_true", "fbp_t"])
+def test_rectangular_images(dev, st, wh, op):
+    """Rows and columns of different lengths, one of them a whole number of 64-pixel lane groups
+    (the weighted line->bin kernel takes its bounds-checked form unless every line is full)."""
+    W, H = wh
+    if st == 2:
+        o = phantom.para_options(48, 97, W, H, 0.01, 0.03, Ang0=0.2, dAng=math.pi / 48, binshift=0.01)
+    else:
+        o = phantom.fan_options(48, 97, W, H, 0.01, 0.012 if st == 1 else 0.06, 3.0, 2.0, st, Ang0=0.2,
+                                binshift=0.01)
+    rng = np.random.default_rng(300 + st)
+    x = inputs(op, o, 2, rng)
+    print(f"[case] rectangular {W}x{H} st={st} op={op}", flush=True)
+    got = gpu_op(op, x, o, dev)
+    ref = oracle_op(op, x.astype(np.float64), o.astype(np.float64))
+    assert rel_l2(got, ref) <= F32_TOL, (op, st, wh, rel_l2(got, ref))
+
+
+@pytest.mark.parametrize("st", [0, 1, 2])
 def test_adjoint_dot(dev, st):
     o = small_options(st, V=64, D=41, W=32, H=24)
     g = O.parse(o)
