@@ -574,11 +574,16 @@ __host__ __device__ inline size_t adj_block_lds(int D, int L) {
 // global -> LDS with global_load_lds_dwordx4 (asynchronous, no register
 // staging) and the next view's LineCoefs of the tile are prefetched into one
 // register per thread while the current view is evaluated; one barrier per view.
-template <typename T, int LPW, int NPE, bool WEIGHTED, int GEO>
+// EXACT: every line has exactly 64*(NPE-1) pixels, so its last pixel edge n is
+// the only edge of group NPE-1; instead of a ninth (or seventeenth) group per
+// line, lane q of the wave evaluates edge n of its line q (LPW <= 64 lanes),
+// one evaluation per view for all the wave's lines.
+template <typename T, int LPW, int NPE, bool WEIGHTED, int GEO, bool EXACT = false>
 __device__ __forceinline__ void adj_block(const DevPlan& P, const double2* __restrict__ tabs, T* __restrict__ outX,
                                           T* __restrict__ outYT, int b, int mode, int tile, double wscale,
                                           unsigned char* __restrict__ smem) {
     constexpr int L = 4 * LPW;
+    constexpr int NG = EXACT ? NPE - 1 : NPE;  // edge groups evaluated per line
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int* vlist = mode == MODE_Y ? P.yviews : P.xviews;
@@ -613,11 +618,13 @@ __device__ __forceinline__ void adj_block(const DevPlan& P, const double2* __res
         return cvalid ? reinterpret_cast<const double*>(P.lines + (size_t)v * P.max_lines + cl)[tid % LC_WORDS] : 0.0;
     };
 
-    double acc[LPW][NPE];
+    double acc[LPW][NG];
 #pragma unroll
     for (int q = 0; q < LPW; q++)
 #pragma unroll
-        for (int i = 0; i < NPE; i++) acc[q][i] = 0.0;
+        for (int i = 0; i < NG; i++) acc[q][i] = 0.0;
+    double accx = 0.0;                          // EXACT: lane q accumulates edge n of line q
+    const int lx = (lane < LPW ? lane : 0) * 4 + wave;  // that line within the tile
 
     table_to_lds(vlist[0], 0);
     if (cthread) scf[tid] = coef_word(vlist[0]);
@@ -637,6 +644,11 @@ __device__ __forceinline__ void adj_block(const DevPlan& P, const double2* __res
         const double* cf = scf + cur * L * LC_WORDS;
         const ViewParams& vp = P.vp[v];
         const Lookup lu = view_lookup(P, vp);
+        double gx = 0.0;  // EXACT: G at edge n of line lx (lanes >= LPW: unused)
+        if (EXACT) {
+            gx = eval_G<GEO>(t, cf[lx * LC_WORDS + LC_A0], cf[lx * LC_WORDS + LC_S], n, lu, hi);
+            if (!WEIGHTED) accx += gx;
+        }
 #pragma unroll
         for (int q = 0; q < LPW; q++) {
             const int lq = q * 4 + wave;  // line within the tile
@@ -648,7 +660,7 @@ __device__ __forceinline__ void adj_block(const DevPlan& P, const double2* __res
             // so their LDS reads overlap.
             if (!WEIGHTED) {
 #pragma unroll
-                for (int i = 0; i < NPE; i++) acc[q][i] += eval_G<GEO>(t, a, s, lane + 64 * i, lu, hi);
+                for (int i = 0; i < NG; i++) acc[q][i] += eval_G<GEO>(t, a, s, lane + 64 * i, lu, hi);
             } else {
                 const double w0 = cf[lq * LC_WORDS + LC_W0], w1 = cf[lq * LC_WORDS + LC_W1];
                 const double w2 = cf[lq * LC_WORDS + LC_W2];
@@ -670,7 +682,7 @@ __device__ __forceinline__ void adj_block(const DevPlan& P, const double2* __res
                 // lane 63 edge 0 of group i; both come from the rotations of the groups
                 double gprev = 0.0, rprev = 0.0;
 #pragma unroll
-                for (int i = 0; i < NPE; i++) {
+                for (int i = 0; i < NG; i++) {
                     const double gi = eval_G<GEO>(t, a, s, lane + 64 * i, lu, hi);
                     const double ri = rot_next_d(gi);
                     if (i > 0) {
@@ -681,8 +693,14 @@ __device__ __forceinline__ void adj_block(const DevPlan& P, const double2* __res
                     gprev = gi;
                     rprev = ri;
                 }
-                // pixels of the last slot group (lane 63's pixel is past the line since 64 * NPE > n)
-                acc[q][NPE - 1] += wgt(lane + 64 * (NPE - 1)) * (rprev - gprev);
+                if (EXACT) {
+                    // last group: lane 63's pixel n-1 ends at edge n, evaluated by lane q
+                    const double gxq = readlane_d(gx, q);
+                    acc[q][NG - 1] += wgt(lane + 64 * (NG - 1)) * ((lane == 63 ? gxq : rprev) - gprev);
+                } else {
+                    // pixels of the last slot group (lane 63's pixel is past the line since 64 * NPE > n)
+                    acc[q][NPE - 1] += wgt(lane + 64 * (NPE - 1)) * (rprev - gprev);
+                }
             }
         }
         if (iv + 1 < nv && cthread) scf[(cur ^ 1) * L * LC_WORDS + tid] = cnext;
@@ -697,7 +715,7 @@ __device__ __forceinline__ void adj_block(const DevPlan& P, const double2* __res
         if (l >= nlines) continue;
         T* dst = mode == MODE_Y ? outYT + ((size_t)b * P.W + l) * P.H : outX + ((size_t)b * P.H + l) * P.W;
 #pragma unroll
-        for (int i = 0; i < NPE; i++) {
+        for (int i = 0; i < NG; i++) {
             if (i >= npe) continue;  // wave-uniform
             const int p = lane + 64 * i;
             double val;
@@ -705,8 +723,9 @@ __device__ __forceinline__ void adj_block(const DevPlan& P, const double2* __res
                 double right = shfl_next_d(acc[q][i]);
                 double r0 = 0.0;
 #pragma unroll
-                for (int k = 0; k < NPE; k++)
+                for (int k = 0; k < NG; k++)
                     if (k == i + 1) r0 = readlane_d(acc[q][k], 0);
+                if (EXACT && i == NG - 1) r0 = readlane_d(accx, q);
                 if (lane == 63) right = r0;
                 val = right - acc[q][i];
             } else {

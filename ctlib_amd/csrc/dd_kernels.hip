@@ -90,7 +90,7 @@ __global__ __launch_bounds__(256) void k_bin_tables(DevPlan P, const TIn* __rest
 // One launch covers both sweep axes: work items [0, B*ntx) are (b, tile of
 // 4*LPW image rows) over the X-axis views, the rest (b, tile of columns) over
 // the Y-axis views, written bottom-to-top to outYT.
-template <typename T, int LPW, int NPE, bool WEIGHTED, int GEO>
+template <typename T, int LPW, int NPE, bool WEIGHTED, int GEO, bool EXACT>
 __global__ __launch_bounds__(256) void k_adj(DevPlan P, const double2* __restrict__ tabs, T* __restrict__ outX,
                                              T* __restrict__ outYT, int B, int ntx, int nty, double wscale) {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -101,7 +101,7 @@ __global__ __launch_bounds__(256) void k_adj(DevPlan P, const double2* __restric
     if (mode == MODE_Y) w -= nwx;
     const int ntiles = mode == MODE_Y ? nty : ntx;
     const int b = w / ntiles, tile = w - b * ntiles;
-    adj_block<T, LPW, NPE, WEIGHTED, GEO>(P, tabs, outX, outYT, b, mode, tile, wscale, smem);
+    adj_block<T, LPW, NPE, WEIGHTED, GEO, EXACT>(P, tabs, outX, outYT, b, mode, tile, wscale, smem);
 }
 
 template <typename T>
@@ -322,7 +322,7 @@ hipError_t launch_bin_tables(const DevPlan& P, int in_dtype, const void* y, cons
 
 namespace {
 
-template <typename T, int LPW, int NPE, bool WTD, int GEO>
+template <typename T, int LPW, int NPE, bool WTD, int GEO, bool EXACT>
 hipError_t adj_launch_t(const DevPlan& P, const double2* tabs, void* outX, void* outYT, int B, double wscale,
                         hipStream_t s) {
     const int L = 4 * LPW;
@@ -330,7 +330,7 @@ hipError_t adj_launch_t(const DevPlan& P, const double2* tabs, void* outX, void*
     const int nty = P.ny ? (P.W + L - 1) / L : 0;
     const int nwork = (ntx + nty) * B;
     const size_t lds = adj_block_lds(P.D, L);
-    auto kern = k_adj<T, LPW, NPE, WTD, GEO>;
+    auto kern = k_adj<T, LPW, NPE, WTD, GEO, EXACT>;
     if (lds > 64 * 1024) {
         hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
@@ -346,10 +346,15 @@ hipError_t adj_npe(const DevPlan& P, const double2* tabs, void* outX, void* outY
     int n = P.nx ? P.W : 0;
     if (P.ny && P.H > n) n = P.H;
     const int npe = (n + 1 + 63) / 64;
-    if (npe <= 5) return adj_launch_t<T, 8, 5, WTD, GEO>(P, tabs, outX, outYT, B, wscale, s);
-    if (npe <= 9) return adj_launch_t<T, 4, 9, WTD, GEO>(P, tabs, outX, outYT, B, wscale, s);
-    if (npe <= 17) return adj_launch_t<T, 2, 17, WTD, GEO>(P, tabs, outX, outYT, B, wscale, s);
-    if (npe <= 33) return adj_launch_t<T, 1, 33, WTD, GEO>(P, tabs, outX, outYT, B, wscale, s);
+    // every line (rows of X views, columns of Y views) exactly 64 * (NPE - 1) pixels
+    auto exact = [&](int NPE) { return (P.nx == 0 || P.W == 64 * (NPE - 1)) && (P.ny == 0 || P.H == 64 * (NPE - 1)); };
+    if (npe <= 5) return exact(5) ? adj_launch_t<T, 8, 5, WTD, GEO, true>(P, tabs, outX, outYT, B, wscale, s)
+                                  : adj_launch_t<T, 8, 5, WTD, GEO, false>(P, tabs, outX, outYT, B, wscale, s);
+    if (npe <= 9) return exact(9) ? adj_launch_t<T, 4, 9, WTD, GEO, true>(P, tabs, outX, outYT, B, wscale, s)
+                                  : adj_launch_t<T, 4, 9, WTD, GEO, false>(P, tabs, outX, outYT, B, wscale, s);
+    if (npe <= 17) return exact(17) ? adj_launch_t<T, 2, 17, WTD, GEO, true>(P, tabs, outX, outYT, B, wscale, s)
+                                    : adj_launch_t<T, 2, 17, WTD, GEO, false>(P, tabs, outX, outYT, B, wscale, s);
+    if (npe <= 33) return adj_launch_t<T, 1, 33, WTD, GEO, false>(P, tabs, outX, outYT, B, wscale, s);
     return hipErrorInvalidValue;
 }
 
