@@ -176,11 +176,12 @@ __device__ __forceinline__ void lent_write(void* p, double c, double s) {
 // Bin values y_k = post_k (S_{k+1} - S_k) from the edge sums of a wave's VPW
 // views, formed with wave shifts; RESID also writes the residual bin tables of
 // projection_t, tabs[b][view] = cumulative {C, g} of g = (y - m) * post.
-template <typename T, int VPW, int NEM, bool RESID>
+// EDGEX: D == 64*NEM, and the sum of the last edge D of view v sits in lane v of Sx.
+template <typename T, int VPW, int NEM, bool RESID, bool EDGEX = false>
 __device__ __forceinline__ void fwd_store(const DevPlan& P, const double (&S)[VPW][NEM], const int (&vw)[VPW], int grp,
                                           int b, int mode, int lane, int wave, T* __restrict__ y,
                                           const double* __restrict__ post, const T* __restrict__ m,
-                                          double2* __restrict__ tabs) {
+                                          double2* __restrict__ tabs, double Sx = 0.0) {
     constexpr int G = 4 * VPW;
     const int nv = mode == MODE_Y ? P.ny : P.nx;
     const int D = P.D;
@@ -205,6 +206,7 @@ __device__ __forceinline__ void fwd_store(const DevPlan& P, const double (&S)[VP
 #pragma unroll
             for (int q = 0; q < NEM; q++)
                 if (q == i + 1) r0 = readlane_d(S[v][q], 0);
+            if (EDGEX && i == NEM - 1) r0 = readlane_d(Sx, v);
             if (lane == 63) nxt = r0;
             double yk = nxt - S[v][i];
             if (pr && k < D) yk *= pr[k];
@@ -287,7 +289,10 @@ __host__ __device__ inline size_t fwd_block_lds(int nmax, int G, bool weighted, 
 // The bin values y_k = post_k (S_{k+1} - S_k) are formed with wave shuffles.
 // RESID: also write the residual bin tables of projection_t (g = (y - m) post).
 // FULL: every line of both axes has exactly 256*M pixels (no pixel predicates).
-template <typename T, int VPW, int NEM, int M, bool WEIGHTED, bool RESID, bool FULL = false>
+// EDGEX (unweighted): D == 64*NEM, so the last bin edge D is the only edge of
+// group NEM; lane v of the wave evaluates it for view v (VPW <= 64), one
+// evaluation per line for all the wave's views instead of a mostly idle group.
+template <typename T, int VPW, int NEM, int M, bool WEIGHTED, bool RESID, bool FULL = false, bool EDGEX = false>
 __device__ __forceinline__ void fwd_block(const DevPlan& P, const T* __restrict__ src, T* __restrict__ y, int b,
                                           int mode, int grp, const double* __restrict__ post,
                                           const T* __restrict__ m, double2* __restrict__ tabs,
@@ -318,6 +323,14 @@ __device__ __forceinline__ void fwd_block(const DevPlan& P, const T* __restrict_
             S[v][i] = 0.0;
         }
     }
+    // EDGEX: lane v (< VPW) owns edge D of view v; the other lanes repeat view 0's
+    const int vxl = lane < VPW ? lane : 0;
+    int vx = vw[0];
+#pragma unroll
+    for (int v = 1; v < VPW; v++)
+        if (vxl == v) vx = vw[v];
+    const double ex = EDGEX ? P.edges[(size_t)vx * (D + 1) + D] : 0.0;
+    double Sx = 0.0;
 
     const int tabn = n + 2;
     LEnt<T>* tab = reinterpret_cast<LEnt<T>*>(smem);                // [2][NTAB][tabn]
@@ -377,6 +390,13 @@ __device__ __forceinline__ void fwd_block(const DevPlan& P, const T* __restrict_
             if (cthread) cnext = crow[(size_t)(l + 1) * LC_WORDS];
         }
         __syncthreads();
+        if (EDGEX) {
+            const int g = wave * VPW + vxl;  // per lane
+            const double tau = __builtin_fma(cfl[g * LC_WORDS + LC_A], ex, cfl[g * LC_WORDS + LC_B]);
+            double c, sl;
+            lent_read(tb + table_index_d(tau, hi), c, sl);
+            Sx = __builtin_fma(cfl[g * LC_WORDS + LC_S], __builtin_fma(tau, sl, c), Sx);
+        }
 #pragma unroll
         for (int v = 0; v < VPW; v++) {
             const int g = wave * VPW + v;
@@ -396,7 +416,7 @@ __device__ __forceinline__ void fwd_block(const DevPlan& P, const T* __restrict_
         }
     }
 
-    fwd_store<T, VPW, NEM, RESID>(P, S, vw, grp, b, mode, lane, wave, y, post, m, tabs);
+    fwd_store<T, VPW, NEM, RESID, EDGEX>(P, S, vw, grp, b, mode, lane, wave, y, post, m, tabs, Sx);
 }
 
 // Entries of a wave-private line table sit at j + j/8 (one 16-byte pad every 8):

@@ -30,7 +30,7 @@ __global__ __launch_bounds__(256) void k_to_ylines(const T* __restrict__ x, T* _
 template <typename T, int NEM, int M, bool WEIGHTED>
 constexpr int fwd_waves() { return sizeof(T) == 4 && !WEIGHTED && NEM > 20 && NEM <= 24 && M <= 4 ? 3 : 1; }
 
-template <typename T, int VPW, int NEM, int M, bool WEIGHTED, bool RESID, bool FULL>
+template <typename T, int VPW, int NEM, int M, bool WEIGHTED, bool RESID, bool FULL, bool EDGEX>
 __global__ __launch_bounds__(256, (fwd_waves<T, NEM, M, WEIGHTED>())) void k_fwd(DevPlan P, const T* __restrict__ srcx, const T* __restrict__ srcy,
                                              T* __restrict__ y, int B, int ngx, int ngy,
                                              const double* __restrict__ post, const T* __restrict__ m,
@@ -43,8 +43,8 @@ __global__ __launch_bounds__(256, (fwd_waves<T, NEM, M, WEIGHTED>())) void k_fwd
     if (mode == MODE_Y) w -= nwx;
     const int ngroups = mode == MODE_Y ? ngy : ngx;
     const int b = w / ngroups, grp = w - b * ngroups;
-    fwd_block<T, VPW, NEM, M, WEIGHTED, RESID, FULL>(P, mode == MODE_Y ? srcy : srcx, y, b, mode, grp, post, m, tabs,
-                                                     smem);
+    fwd_block<T, VPW, NEM, M, WEIGHTED, RESID, FULL, EDGEX>(P, mode == MODE_Y ? srcy : srcx, y, b, mode, grp, post, m,
+                                                            tabs, smem);
 }
 
 // Per-(b, view) bin tables {C_j, g_j} of g_k = ((y - m)[o(k)] * pre[k]) * scale, j = 0..D+1.
@@ -229,13 +229,13 @@ struct FwdArgs {
     double2* tabs;
 };
 
-template <typename T, int VPW, int NEM, int M, bool WTD, bool RESID, bool FULL>
+template <typename T, int VPW, int NEM, int M, bool WTD, bool RESID, bool FULL, bool EDGEX>
 hipError_t fwd_launch_t(const DevPlan& P, const FwdArgs& a, hipStream_t s) {
     constexpr int G = 4 * VPW;
     const int ngx = (P.nx + G - 1) / G, ngy = (P.ny + G - 1) / G;
     const int nwork = (ngx + ngy) * a.B;
     const size_t lds = fwd_lds_bytes(P, WTD, G);
-    auto kern = k_fwd<T, VPW, NEM, M, WTD, RESID, FULL>;
+    auto kern = k_fwd<T, VPW, NEM, M, WTD, RESID, FULL, EDGEX>;
     if (lds > 64 * 1024) {
         hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
@@ -246,29 +246,34 @@ hipError_t fwd_launch_t(const DevPlan& P, const FwdArgs& a, hipStream_t s) {
 }
 
 // Pixels per thread of the line table (M >= n / 256).
-template <typename T, int VPW, int NEM, bool WTD, bool RESID>
+template <typename T, int VPW, int NEM, bool WTD, bool RESID, bool EDGEX = false>
 hipError_t fwd_m(const DevPlan& P, const FwdArgs& a, hipStream_t s) {
     int n = P.nx ? P.W : 0;
     if (P.ny && P.H > n) n = P.H;
     const int m = (n + NT - 1) / NT;
     // every swept line (rows of X views, columns of Y views) exactly NT * M pixels
     auto full = [&](int M) { return (P.nx == 0 || P.W == NT * M) && (P.ny == 0 || P.H == NT * M); };
-    if (m <= 2) return full(2) ? fwd_launch_t<T, VPW, NEM, 2, WTD, RESID, true>(P, a, s)
-                               : fwd_launch_t<T, VPW, NEM, 2, WTD, RESID, false>(P, a, s);
-    if (m <= 4) return full(4) ? fwd_launch_t<T, VPW, NEM, 4, WTD, RESID, true>(P, a, s)
-                               : fwd_launch_t<T, VPW, NEM, 4, WTD, RESID, false>(P, a, s);
-    if (m <= 8) return fwd_launch_t<T, VPW, NEM, 8, WTD, RESID, false>(P, a, s);
+    if (m <= 2) return full(2) ? fwd_launch_t<T, VPW, NEM, 2, WTD, RESID, true, EDGEX>(P, a, s)
+                               : fwd_launch_t<T, VPW, NEM, 2, WTD, RESID, false, EDGEX>(P, a, s);
+    if (m <= 4) return full(4) ? fwd_launch_t<T, VPW, NEM, 4, WTD, RESID, true, EDGEX>(P, a, s)
+                               : fwd_launch_t<T, VPW, NEM, 4, WTD, RESID, false, EDGEX>(P, a, s);
+    if (m <= 8) return fwd_launch_t<T, VPW, NEM, 8, WTD, RESID, false, EDGEX>(P, a, s);
     return hipErrorInvalidValue;
 }
 
 // Edge slots of 64 per wave and view: NEM >= ceil((D+1)/64); two views per wave only up to NEM 12.
+// D = 512, 1024 or 1536 bins (unweighted): the last edge D is evaluated packed (EDGEX).
 template <typename T, int VPW, bool WTD, bool RESID>
 hipError_t fwd_ne(const DevPlan& P, const FwdArgs& a, hipStream_t s) {
     const int ne = (P.D + 1 + 63) / 64;
+    if (!WTD && P.D == 512) return fwd_m<T, VPW, 8, WTD, RESID, true>(P, a, s);
+    if (!WTD && VPW == 1 && P.D == 1024) return fwd_m<T, 1, 16, WTD, RESID, true>(P, a, s);
+    if (!WTD && VPW == 1 && P.D == 1536) return fwd_m<T, 1, 24, WTD, RESID, true>(P, a, s);
     if (ne <= 4) return fwd_m<T, VPW, 4, WTD, RESID>(P, a, s);
     if (ne <= 8) return fwd_m<T, VPW, 8, WTD, RESID>(P, a, s);
     if (ne <= 12) return fwd_m<T, VPW, 12, WTD, RESID>(P, a, s);
     if (VPW != 1) return hipErrorInvalidValue;
+    if (ne <= 14) return fwd_m<T, 1, 14, WTD, RESID>(P, a, s);
     if (ne <= 16) return fwd_m<T, 1, 16, WTD, RESID>(P, a, s);
     if (ne <= 20) return fwd_m<T, 1, 20, WTD, RESID>(P, a, s);
     if (ne <= 24) return fwd_m<T, 1, 24, WTD, RESID>(P, a, s);

@@ -141,6 +141,23 @@ def test_rectangular_images(dev, st, wh, op):
     assert rel_l2(got, ref) <= F32_TOL, (op, st, wh, rel_l2(got, ref))
 
 
+@pytest.mark.parametrize("D", [512, 1024, 1536])
+@pytest.mark.parametrize("st", [0, 2])
+@pytest.mark.parametrize("op", ["projection", "backprojection_t"])
+def test_detector_multiple_of_64(dev, D, st, op):
+    """Detectors of 64*k bins (their last edge is evaluated apart from the 64-edge groups)."""
+    if st == 2:
+        o = phantom.para_options(20, D, 64, 64, 0.02, 1.6 / D, Ang0=0.3, dAng=math.pi / 20, binshift=0.001)
+    else:
+        o = phantom.fan_options(20, D, 64, 64, 0.02, 2.6 / D, 3.0, 2.0, st, Ang0=0.3, binshift=0.001)
+    rng = np.random.default_rng(400 + D + st)
+    x = inputs(op, o, 2, rng)
+    print(f"[case] D={D} st={st} op={op}", flush=True)
+    got = gpu_op(op, x, o, dev)
+    ref = oracle_op(op, x.astype(np.float64), o.astype(np.float64))
+    assert rel_l2(got, ref) <= F32_TOL, (op, st, D, rel_l2(got, ref))
+
+
 @pytest.mark.parametrize("st", [0, 1, 2])
 def test_adjoint_dot(dev, st):
     o = small_options(st, V=64, D=41, W=32, H=24)
