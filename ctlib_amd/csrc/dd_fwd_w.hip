@@ -89,7 +89,9 @@ hipError_t by_nem(const DevPlan& P, const Args& a, hipStream_t s) {
     if (ne <= 4) return by_ppl<T, 4>(P, a, s);
     if (ne <= 8) return by_ppl<T, 8>(P, a, s);
     if (ne <= 12) return by_ppl<T, 12>(P, a, s);
+    if (ne <= 14) return by_ppl<T, 14>(P, a, s);
     if (ne <= 16) return by_ppl<T, 16>(P, a, s);
+    if (ne <= 20) return by_ppl<T, 20>(P, a, s);
     if (ne <= 24) return by_ppl<T, 24>(P, a, s);
     if (ne <= 32) return by_ppl<T, 32>(P, a, s);
     return hipErrorInvalidValue;
