@@ -141,6 +141,24 @@ def test_rectangular_images(dev, st, wh, op):
     assert rel_l2(got, ref) <= F32_TOL, (op, st, wh, rel_l2(got, ref))
 
 
+@pytest.mark.parametrize("st", [0, 1, 2])
+@pytest.mark.parametrize("op", ["backprojection_t", "backprojection_t_true", "fbp_t", "projection"])
+def test_lines_longer_than_1024(dev, st, op):
+    """Rows of 1152 pixels: the weighted line->bin operators take the shared-table kernel with one
+    table per view (the wave-private-table kernel covers lines of up to 1024 pixels)."""
+    if st == 2:
+        o = phantom.para_options(12, 97, 1152, 48, 0.002, 0.03, Ang0=0.2, dAng=math.pi / 12, binshift=0.01)
+    else:
+        o = phantom.fan_options(12, 97, 1152, 48, 0.002, 0.012 if st == 1 else 0.06, 3.0, 2.0, st, Ang0=0.2,
+                                binshift=0.01)
+    rng = np.random.default_rng(500 + st)
+    x = inputs(op, o, 2, rng)
+    print(f"[case] 1152-pixel rows st={st} op={op}", flush=True)
+    got = gpu_op(op, x, o, dev)
+    ref = oracle_op(op, x.astype(np.float64), o.astype(np.float64))
+    assert rel_l2(got, ref) <= F32_TOL, (op, st, rel_l2(got, ref))
+
+
 @pytest.mark.parametrize("D", [512, 1024, 1536])
 @pytest.mark.parametrize("st", [0, 2])
 @pytest.mark.parametrize("op", ["projection", "backprojection_t"])
