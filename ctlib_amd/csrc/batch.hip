@@ -46,10 +46,11 @@ struct SliceDev {
     void* xT;
     double2* tabs;
     void* yt;
-    int fcls;   // forward variant: edge-slot class
-    int acls;   // adjoint variant: NPE class * 4 + GEO (GEO_EAP for fan-EA with the polynomial atan)
+    int fcls;   // forward variant (edge-slot class, see run_fwd)
+    int acls;   // adjoint variant: line class * 4 + GEO (GEO_EAP for fan-EA with the polynomial atan)
     int ngx, ngy, ntx, nty, nband;
-    int pad[3];
+    int tileL;  // image lines per adjoint item
+    int pad[2];
 };
 
 struct Item {
@@ -62,25 +63,44 @@ __device__ __forceinline__ void run_fwd(const SliceDev& sd, int mode, int grp, u
     const T* src = (const T*)(mode == MODE_Y ? sd.xT : sd.x);
     T* y = (T*)sd.y;
     const T* m = (const T*)sd.m;
-    switch (sd.fcls) {  // edge slots per view: 12 (D < 768) or 20 (D < 1280); M = 2 (lines <= 512)
+    // edge slots per view: 12 (D < 768) or 20 (D < 1280); 512 or 1024 bins: 8 or 16 slots and the last
+    // edge packed; M = 2 (lines <= 512)
+    switch (sd.fcls) {
     case 0: fwd_block<T, 1, 12, 2, false, true>(sd.P, src, y, 0, mode, grp, sd.P.post_prj, m, sd.tabs, smem); break;
+    case 2:
+        fwd_block<T, 1, 8, 2, false, true, false, true>(sd.P, src, y, 0, mode, grp, sd.P.post_prj, m, sd.tabs, smem);
+        break;
+    case 3:
+        fwd_block<T, 1, 16, 2, false, true, false, true>(sd.P, src, y, 0, mode, grp, sd.P.post_prj, m, sd.tabs, smem);
+        break;
     default: fwd_block<T, 1, 20, 2, false, true>(sd.P, src, y, 0, mode, grp, sd.P.post_prj, m, sd.tabs, smem); break;
     }
 }
 
-template <typename T>
-__device__ __forceinline__ void run_adj(const SliceDev& sd, int mode, int tile, unsigned char* smem) {
+template <typename T, int LPW, int NPE, bool EXACT>
+__device__ __forceinline__ void run_adj_geo(const SliceDev& sd, int mode, int tile, unsigned char* smem) {
     T* ox = (T*)sd.grad;
     T* oy = (T*)sd.yt;
-    switch (sd.acls) {
-    case 0: adj_block<T, 8, 5, false, GEO_ED>(sd.P, sd.tabs, ox, oy, 0, mode, tile, 1.0, smem); break;
-    case 1: adj_block<T, 8, 5, false, GEO_EA>(sd.P, sd.tabs, ox, oy, 0, mode, tile, 1.0, smem); break;
-    case 2: adj_block<T, 8, 5, false, GEO_PARA>(sd.P, sd.tabs, ox, oy, 0, mode, tile, 1.0, smem); break;
-    case 3: adj_block<T, 8, 5, false, GEO_EAP>(sd.P, sd.tabs, ox, oy, 0, mode, tile, 1.0, smem); break;
-    case 4: adj_block<T, 4, 9, false, GEO_ED>(sd.P, sd.tabs, ox, oy, 0, mode, tile, 1.0, smem); break;
-    case 5: adj_block<T, 4, 9, false, GEO_EA>(sd.P, sd.tabs, ox, oy, 0, mode, tile, 1.0, smem); break;
-    case 6: adj_block<T, 4, 9, false, GEO_PARA>(sd.P, sd.tabs, ox, oy, 0, mode, tile, 1.0, smem); break;
-    default: adj_block<T, 4, 9, false, GEO_EAP>(sd.P, sd.tabs, ox, oy, 0, mode, tile, 1.0, smem); break;
+    switch (sd.acls & 3) {
+    case GEO_ED: adj_block<T, LPW, NPE, false, GEO_ED, EXACT>(sd.P, sd.tabs, ox, oy, 0, mode, tile, 1.0, smem); break;
+    case GEO_EA: adj_block<T, LPW, NPE, false, GEO_EA, EXACT>(sd.P, sd.tabs, ox, oy, 0, mode, tile, 1.0, smem); break;
+    case GEO_PARA:
+        adj_block<T, LPW, NPE, false, GEO_PARA, EXACT>(sd.P, sd.tabs, ox, oy, 0, mode, tile, 1.0, smem);
+        break;
+    default: adj_block<T, LPW, NPE, false, GEO_EAP, EXACT>(sd.P, sd.tabs, ox, oy, 0, mode, tile, 1.0, smem); break;
+    }
+}
+
+// Line classes: 0 lines <= 256 px, 1 lines <= 512 px; all lines exactly 2: 256, 3: 512, 4: 384 px
+// (the last pixel edge evaluated packed).
+template <typename T>
+__device__ __forceinline__ void run_adj(const SliceDev& sd, int mode, int tile, unsigned char* smem) {
+    switch (sd.acls >> 2) {
+    case 0: run_adj_geo<T, 8, 5, false>(sd, mode, tile, smem); break;
+    case 2: run_adj_geo<T, 8, 5, true>(sd, mode, tile, smem); break;
+    case 3: run_adj_geo<T, 4, 9, true>(sd, mode, tile, smem); break;
+    case 4: run_adj_geo<T, 4, 7, true>(sd, mode, tile, smem); break;
+    default: run_adj_geo<T, 4, 9, false>(sd, mode, tile, smem); break;
     }
 }
 
@@ -231,7 +251,7 @@ void build_items(ctlib_batch* b) {
             ph[IT_F].push_back({(double)P.H * (P.D + 1) * BG, Item{IT_F, s, MODE_X, g}});
         for (int g = 0; g < sd.ngy; g++)
             ph[IT_F].push_back({(double)P.W * (P.D + 1) * BG, Item{IT_F, s, MODE_Y, g}});
-        const int L = (sd.acls < 4) ? 32 : 16;
+        const int L = sd.tileL;
         for (int t = 0; t < sd.ntx; t++) ph[IT_A].push_back({(double)L * (P.W + 1) * P.nx, Item{IT_A, s, MODE_X, t}});
         for (int t = 0; t < sd.nty; t++) ph[IT_A].push_back({(double)L * (P.H + 1) * P.ny, Item{IT_A, s, MODE_Y, t}});
         if (P.ny)
@@ -288,11 +308,14 @@ int ctlib_batch_create(const ctlib_geom* geoms, int nslices, int device, int dty
         const DevPlan& P = sd.P;
         const int ne = (P.D + 1 + 63) / 64;
         const int nmax = std::max(P.nx ? P.W : 0, P.ny ? P.H : 0);
-        sd.fcls = ne <= 12 ? 0 : 1;
+        sd.fcls = P.D == 512 ? 2 : P.D == 1024 ? 3 : ne <= 12 ? 0 : 1;
         const int npe = (nmax + 1 + 63) / 64;
-        const int npecls = npe <= 5 ? 0 : 1;
-        sd.acls = npecls * 4 + (P.scan_type == GEO_EA && P.ea_poly ? GEO_EAP : P.scan_type);
-        const int L = npecls == 0 ? 32 : 16;
+        // every line (rows of X views, columns of Y views) exactly n pixels
+        auto all_lines = [&](int n) { return (P.nx == 0 || P.W == n) && (P.ny == 0 || P.H == n); };
+        const int lcls = all_lines(256) ? 2 : all_lines(512) ? 3 : all_lines(384) ? 4 : npe <= 5 ? 0 : 1;
+        sd.acls = lcls * 4 + (P.scan_type == GEO_EA && P.ea_poly ? GEO_EAP : P.scan_type);
+        const int L = lcls == 0 || lcls == 2 ? 32 : 16;
+        sd.tileL = L;
         sd.ngx = (P.nx + BG - 1) / BG;
         sd.ngy = (P.ny + BG - 1) / BG;
         sd.ntx = P.nx ? (P.H + L - 1) / L : 0;
