@@ -62,6 +62,7 @@ def build(arch="gfx950", verbose=False, with_torch=True):
                    "-c", os.path.join(CSRC, "plan.cpp")],
         "abi.o": [HIPCC, f"--offload-arch={arch}", *HIP_FLAGS, "-c", os.path.join(CSRC, "abi.cpp")],
         "dd_kernels.o": [HIPCC, f"--offload-arch={arch}", *HIP_FLAGS, "-c", os.path.join(CSRC, "dd_kernels.hip")],
+        "dd_adj.o": [HIPCC, f"--offload-arch={arch}", *HIP_FLAGS, "-c", os.path.join(CSRC, "dd_adj.hip")],
         "dd_fwd_w.o": [HIPCC, f"--offload-arch={arch}", *HIP_FLAGS, "-c", os.path.join(CSRC, "dd_fwd_w.hip")],
         "ramp_mfma.o": [HIPCC, f"--offload-arch={arch}", *HIP_FLAGS, "-c", os.path.join(CSRC, "ramp_mfma.hip")],
         "batch.o": [HIPCC, f"--offload-arch={arch}", *HIP_FLAGS, "-c", os.path.join(CSRC, "batch.hip")],

@@ -1,6 +1,9 @@
-// Stand-alone launches of the distance-driven projector kernels (gfx950).
-// The per-work-item device code lives in dd_device.h; the persistent
-// heterogeneous batch kernel (batch.hip) runs the same block functions.
+// Stand-alone launches of the line -> bin kernels (gfx950): projection,
+// backprojection_t and the residual-fused forward (k_fwd), the per-(slice,
+// view) bin tables of the adjoints, and the Y-axis line copy / combine
+// transposes.  The per-work-item device code lives in dd_device.h; the
+// persistent heterogeneous batch kernel (batch.hip) runs the same block
+// functions, and the bin -> pixel launches live in dd_adj.hip.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -87,23 +90,6 @@ __global__ __launch_bounds__(256) void k_bin_tables(DevPlan P, const TIn* __rest
     }
 }
 
-// One launch covers both sweep axes: work items [0, B*ntx) are (b, tile of
-// 4*LPW image rows) over the X-axis views, the rest (b, tile of columns) over
-// the Y-axis views, written bottom-to-top to outYT.
-template <typename T, int LPW, int NPE, bool WEIGHTED, int GEO, bool EXACT>
-__global__ __launch_bounds__(256) void k_adj(DevPlan P, const double2* __restrict__ tabs, T* __restrict__ outX,
-                                             T* __restrict__ outYT, int B, int ntx, int nty, double wscale) {
-    extern __shared__ __align__(16) unsigned char smem[];
-    const int nwx = B * ntx;
-    int w = xcd_remap(blockIdx.x, gridDim.x, nwx + B * nty);
-    if (w < 0) return;
-    const int mode = w < nwx ? MODE_X : MODE_Y;
-    if (mode == MODE_Y) w -= nwx;
-    const int ntiles = mode == MODE_Y ? nty : ntx;
-    const int b = w / ntiles, tile = w - b * ntiles;
-    adj_block<T, LPW, NPE, WEIGHTED, GEO, EXACT>(P, tabs, outX, outYT, b, mode, tile, wscale, smem);
-}
-
 template <typename T>
 __global__ __launch_bounds__(256) void k_combine(T* __restrict__ out, const T* __restrict__ yt, int H, int W, int hasX,
                                                  int hasY) {
@@ -113,88 +99,6 @@ __global__ __launch_bounds__(256) void k_combine(T* __restrict__ out, const T* _
                     tile);
 }
 
-
-// backprojection_sv: one view's term of the weighted backprojection, written to
-// its own image, out[b][v][r][c] (fan_ed_kernel.cu:945-1161).  One block per
-// (b, view); the view's bin table is staged in LDS.  X-axis views: a wave owns
-// an image row and its lanes the pixel edges of 64 consecutive columns (NPE
-// groups, all evaluated before the stores; the right edge comes from the
-// neighbouring lane), so stores are row-contiguous.  Y-axis views: each lane
-// owns a column and walks its pixels bottom to top, SV_UNROLL at a time, so
-// the 64 lanes of a wave store 64 adjacent columns of one row.
-constexpr int SV_UNROLL = 8;
-
-template <typename T, int GEO, int NPE>
-__global__ __launch_bounds__(256) void k_bprj_sv(DevPlan P, const double2* __restrict__ tabs, T* __restrict__ out,
-                                                 int nitems, double wscale) {
-    extern __shared__ __align__(16) unsigned char smem[];
-    const int item = xcd_remap(blockIdx.x, gridDim.x, nitems);  // b * V + v
-    if (item < 0) return;
-    const int v = item % P.V;
-    const int D = P.D, H = P.H, W = P.W;
-    const int tabn = D + 2;
-    double2* t = reinterpret_cast<double2*>(smem);
-    const double2* src = tabs + (size_t)item * tabn;
-    for (int j = threadIdx.x; j < tabn; j += NT) t[j] = src[j];
-    __syncthreads();
-    const int lane = threadIdx.x & 63;
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const ViewParams& vp = P.vp[v];
-    const Lookup lu = view_lookup(P, vp);
-    const float hi = (float)(D + 1);
-    const LineCoef* lc = P.lines + (size_t)v * P.max_lines;
-    T* img = out + (size_t)item * H * W;
-    auto wgt = [&](const LineCoef& c, int p) {
-        const double u = __builtin_fma(c.w1, (double)p, c.w0);
-        const double d = __builtin_fma(u, u, c.w2);
-        return c.inv_s * wscale * (double)__builtin_amdgcn_rcpf((float)d);
-    };
-    if (vp.xmode) {
-        for (int r = wave; r < H; r += NT / 64) {
-            const LineCoef c = lc[r];
-            T* dst = img + (size_t)r * W;
-            double g[NPE];
-#pragma unroll
-            for (int i = 0; i < NPE; i++) g[i] = eval_G<GEO>(t, c.a, c.s, lane + 64 * i, lu, hi);
-#pragma unroll
-            for (int i = 0; i < NPE; i++) {
-                double right = shfl_next_d(g[i]);
-                if (i + 1 < NPE) {
-                    const double r0 = readlane_d(g[i + 1], 0);
-                    if (lane == 63) right = r0;
-                }
-                // lane 63 of the last group is past the row: 64 * NPE > W
-                const int p = lane + 64 * i;
-                if (p < W) dst[p] = (T)(wgt(c, p) * (right - g[i]));
-            }
-        }
-    } else {
-        for (int c0 = wave * 64; c0 < W; c0 += NT) {
-            const int col = c0 + lane;
-            const bool ok = col < W;
-            const LineCoef c = lc[ok ? col : W - 1];
-            double gprev = eval_G<GEO>(t, c.a, c.s, 0, lu, hi);
-            T* dst = img + (size_t)(H - 1) * W + col;
-            int i = 0;
-            for (; i + SV_UNROLL <= H; i += SV_UNROLL) {
-                double g[SV_UNROLL];
-#pragma unroll
-                for (int u = 0; u < SV_UNROLL; u++) g[u] = eval_G<GEO>(t, c.a, c.s, i + u + 1, lu, hi);
-#pragma unroll
-                for (int u = 0; u < SV_UNROLL; u++) {
-                    const double left = u ? g[u - 1] : gprev;
-                    if (ok) dst[-(ptrdiff_t)(i + u) * W] = (T)(wgt(c, i + u) * (g[u] - left));
-                }
-                gprev = g[SV_UNROLL - 1];
-            }
-            for (; i < H; i++) {
-                const double gi = eval_G<GEO>(t, c.a, c.s, i + 1, lu, hi);
-                if (ok) dst[-(ptrdiff_t)i * W] = (T)(wgt(c, i) * (gi - gprev));
-                gprev = gi;
-            }
-        }
-    }
-}
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -323,113 +227,6 @@ hipError_t launch_bin_tables(const DevPlan& P, int in_dtype, const void* y, cons
         hipLaunchKernelGGL(k_bin_tables<double>, grid, dim3(256), 0, s, P, (const double*)y, (const double*)m, pre,
                            scale, tabs, nrows);
     return hipGetLastError();
-}
-
-namespace {
-
-template <typename T, int LPW, int NPE, bool WTD, int GEO, bool EXACT>
-hipError_t adj_launch_t(const DevPlan& P, const double2* tabs, void* outX, void* outYT, int B, double wscale,
-                        hipStream_t s) {
-    const int L = 4 * LPW;
-    const int ntx = P.nx ? (P.H + L - 1) / L : 0;
-    const int nty = P.ny ? (P.W + L - 1) / L : 0;
-    const int nwork = (ntx + nty) * B;
-    const size_t lds = adj_block_lds(P.D, L);
-    auto kern = k_adj<T, LPW, NPE, WTD, GEO, EXACT>;
-    if (lds > 64 * 1024) {
-        hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        if (e != hipSuccess) return e;
-    }
-    hipLaunchKernelGGL(kern, dim3(grid_for(nwork)), dim3(NT), lds, s, P, tabs, (T*)outX, (T*)outYT, B, ntx, nty,
-                       wscale);
-    return hipGetLastError();
-}
-
-template <typename T, bool WTD, int GEO>
-hipError_t adj_npe(const DevPlan& P, const double2* tabs, void* outX, void* outYT, int B, double wscale,
-                   hipStream_t s) {
-    int n = P.nx ? P.W : 0;
-    if (P.ny && P.H > n) n = P.H;
-    const int npe = (n + 1 + 63) / 64;
-    // every line (rows of X views, columns of Y views) exactly 64 * (NPE - 1) pixels
-    auto exact = [&](int NPE) { return (P.nx == 0 || P.W == 64 * (NPE - 1)) && (P.ny == 0 || P.H == 64 * (NPE - 1)); };
-    if (npe <= 5) return exact(5) ? adj_launch_t<T, 8, 5, WTD, GEO, true>(P, tabs, outX, outYT, B, wscale, s)
-                                  : adj_launch_t<T, 8, 5, WTD, GEO, false>(P, tabs, outX, outYT, B, wscale, s);
-    if (npe <= 9) return exact(9) ? adj_launch_t<T, 4, 9, WTD, GEO, true>(P, tabs, outX, outYT, B, wscale, s)
-                                  : adj_launch_t<T, 4, 9, WTD, GEO, false>(P, tabs, outX, outYT, B, wscale, s);
-    if (npe <= 17) return exact(17) ? adj_launch_t<T, 2, 17, WTD, GEO, true>(P, tabs, outX, outYT, B, wscale, s)
-                                    : adj_launch_t<T, 2, 17, WTD, GEO, false>(P, tabs, outX, outYT, B, wscale, s);
-    if (npe <= 33) return adj_launch_t<T, 1, 33, WTD, GEO, false>(P, tabs, outX, outYT, B, wscale, s);
-    return hipErrorInvalidValue;
-}
-
-template <typename T, bool WTD>
-hipError_t adj_geo(const DevPlan& P, const double2* tabs, void* outX, void* outYT, int B, double wscale,
-                   hipStream_t s) {
-    switch (P.scan_type) {
-    case GEO_ED: return adj_npe<T, WTD, GEO_ED>(P, tabs, outX, outYT, B, wscale, s);
-    case GEO_EA:
-        return P.ea_poly ? adj_npe<T, WTD, GEO_EAP>(P, tabs, outX, outYT, B, wscale, s)
-                         : adj_npe<T, WTD, GEO_EA>(P, tabs, outX, outYT, B, wscale, s);
-    default: return adj_npe<T, WTD, GEO_PARA>(P, tabs, outX, outYT, B, wscale, s);
-    }
-}
-
-}  // namespace
-
-hipError_t launch_adj(const DevPlan& P, int dtype, const double2* tabs, void* outX, void* outYT, int B, bool weighted,
-                      double wscale, AdjShape, hipStream_t s) {
-    if (B <= 0 || P.nx + P.ny == 0) return hipSuccess;
-    if (dtype == DT_F32) {
-        return weighted ? adj_geo<float, true>(P, tabs, outX, outYT, B, wscale, s)
-                        : adj_geo<float, false>(P, tabs, outX, outYT, B, wscale, s);
-    }
-    return weighted ? adj_geo<double, true>(P, tabs, outX, outYT, B, wscale, s)
-                    : adj_geo<double, false>(P, tabs, outX, outYT, B, wscale, s);
-}
-
-namespace {
-
-template <typename T, int GEO, int NPE>
-hipError_t sv_launch_npe(const DevPlan& P, const double2* tabs, void* out, int B, double wscale, hipStream_t s) {
-    const int nitems = B * P.V;
-    const size_t lds = (size_t)(P.D + 2) * sizeof(double2);
-    hipLaunchKernelGGL((k_bprj_sv<T, GEO, NPE>), dim3(grid_for(nitems)), dim3(NT), lds, s, P, tabs, (T*)out, nitems,
-                       wscale);
-    return hipGetLastError();
-}
-
-// Edge groups of an X-axis row: 64 * NPE >= W + 1.
-template <typename T, int GEO>
-hipError_t sv_launch_t(const DevPlan& P, const double2* tabs, void* out, int B, double wscale, hipStream_t s) {
-    const int npe = (P.W + 1 + 63) / 64;
-    if (npe <= 5) return sv_launch_npe<T, GEO, 5>(P, tabs, out, B, wscale, s);
-    if (npe <= 9) return sv_launch_npe<T, GEO, 9>(P, tabs, out, B, wscale, s);
-    if (npe <= 17) return sv_launch_npe<T, GEO, 17>(P, tabs, out, B, wscale, s);
-    if (npe <= 33) return sv_launch_npe<T, GEO, 33>(P, tabs, out, B, wscale, s);
-    return hipErrorInvalidValue;
-}
-
-template <typename T>
-hipError_t sv_geo(const DevPlan& P, const double2* tabs, void* out, int B, double wscale, hipStream_t s) {
-    switch (P.scan_type) {
-    case GEO_ED: return sv_launch_t<T, GEO_ED>(P, tabs, out, B, wscale, s);
-    case GEO_EA:
-        return P.ea_poly ? sv_launch_t<T, GEO_EAP>(P, tabs, out, B, wscale, s)
-                         : sv_launch_t<T, GEO_EA>(P, tabs, out, B, wscale, s);
-    default: return sv_launch_t<T, GEO_PARA>(P, tabs, out, B, wscale, s);
-    }
-}
-
-}  // namespace
-
-hipError_t launch_bprj_sv(const DevPlan& P, int dtype, const double2* tabs, void* out, int B, double wscale,
-                          hipStream_t s) {
-    if (B <= 0) return hipSuccess;
-    if ((size_t)B * P.V > (size_t)INT32_MAX || (size_t)(P.D + 2) * sizeof(double2) > 64 * 1024)
-        return hipErrorInvalidValue;
-    if (dtype == DT_F32) return sv_geo<float>(P, tabs, out, B, wscale, s);
-    return sv_geo<double>(P, tabs, out, B, wscale, s);
 }
 
 hipError_t launch_combine(void* out, const void* yt, int dtype, int B, int H, int W, bool hasX, bool hasY,

@@ -159,6 +159,22 @@ def test_lines_longer_than_1024(dev, st, op):
     assert rel_l2(got, ref) <= F32_TOL, (op, st, rel_l2(got, ref))
 
 
+@pytest.mark.parametrize("n", [384, 400, 768])
+@pytest.mark.parametrize("st", [0, 1])
+@pytest.mark.parametrize("op", ["projection_t", "backprojection"])
+def test_square_line_lengths(dev, n, st, op):
+    """384- and 768-pixel lines (a whole number of 64-pixel groups: the last pixel edge is evaluated
+    packed) and 400-pixel lines (a partly filled last group)."""
+    o = phantom.fan_options(12, 97, n, n, 2.3 / n, 0.012 if st == 1 else 0.06, 3.0, 2.0, st, Ang0=0.2,
+                            binshift=0.01)
+    rng = np.random.default_rng(600 + n + st)
+    x = inputs(op, o, 1, rng)
+    print(f"[case] {n}^2 st={st} op={op}", flush=True)
+    got = gpu_op(op, x, o, dev)
+    ref = oracle_op(op, x.astype(np.float64), o.astype(np.float64))
+    assert rel_l2(got, ref) <= F32_TOL, (op, st, n, rel_l2(got, ref))
+
+
 @pytest.mark.parametrize("D", [512, 1024, 1536])
 @pytest.mark.parametrize("st", [0, 2])
 @pytest.mark.parametrize("op", ["projection", "backprojection_t"])
