@@ -146,9 +146,10 @@ def workload_hetero(args):
 
 
 def workload_ops(args):
-    """Every public operator on the reference's fan-ED configuration (512^2, 736 x 736, B=8) and on
-    fan-EA (512^2, 1024 x 888, B=16): call time, sweep-model rate of its DD kernel (library events),
-    and the FP64 oracle on the host for the same operator on a bounded sample of the views."""
+    """Every public operator on the reference's fan-ED configuration (512^2, 736 x 736, B=8; fp32 and
+    fp64 data) and on fan-EA (512^2, 1024 x 888, B=16): call time, sweep-model rate of its DD kernel
+    (library events; element size 4 or 8 bytes), and the FP64 oracle on the host for the same
+    operator on a bounded sample of the views."""
     import torch
 
     import ctlib_amd
@@ -159,14 +160,15 @@ def workload_ops(args):
     names = ("projection", "projection_t", "backprojection", "backprojection_t", "fbp")
     img_in = ("projection", "backprojection_t")
     out = {"metric": "per-operator calls/s and sweep-model %HBM", "n_gpus": 1, "steps": args.steps,
-           "warmup": args.warmup, "dtype": "f32", "data": "synthetic", "configs": {}}
-    for cfg, B in ((1, 8), (2, 16)):
+           "warmup": args.warmup, "dtype": "f32 (and f64 where marked)", "data": "synthetic", "configs": {}}
+    for cfg, B, dt in ((1, 8, torch.float32), (2, 16, torch.float32), (1, 8, torch.float64)):
         o = phantom.CONFIGS[cfg]["options"]
-        ot = torch.from_numpy(o)
+        ot = torch.from_numpy(o).to(dt)
         g = ctlib_amd.parse_options(ot)
-        x = torch.from_numpy(phantom.phantom_batch(B, g["height"], g["width"], config=cfg, scale=0.02)).to(dev)
+        x = torch.from_numpy(phantom.phantom_batch(B, g["height"], g["width"], config=cfg, scale=0.02)).to(dev, dt)
         y = ctlib_amd.projection(x, ot)
-        sweep = 4.0 * B * g["views"] * (g["height"] * g["width"] + g["dets"])
+        es = 4.0 if dt == torch.float32 else 8.0
+        sweep = es * B * g["views"] * (g["height"] * g["width"] + g["dets"])
         rows = {}
         for name in names:
             f, inp = getattr(ctlib_amd, name), (x if name in img_in else y)
@@ -191,7 +193,7 @@ def workload_ops(args):
                 row["ramp_ms"] = round(r_ms, 4)
                 row["ramp_frac_fp64_mfma"] = round(st["ramp"]["flops"] / max(st["ramp"]["launches"], 1)
                                                    / (r_ms * 1e-3) / 78.6e12, 4)
-            if not args.no_cpu:
+            if not args.no_cpu and dt == torch.float32:
                 # FP64 oracle, same operator, one slice over a view subset, scaled to the call
                 vs = max(8, g["views"] // 16)
                 o_sub = o.astype(np.float64).copy()
@@ -203,7 +205,8 @@ def workload_ops(args):
                 row["cpu_oracle_calls_per_s"] = round(1.0 / (dt * (g["views"] / vs) * B), 5)
                 row["cpu_sample"] = f"1 slice x {vs} views, {O.max_threads()} threads, scaled"
             rows[name] = row
-        out["configs"][f"{'fan-ED' if cfg == 1 else 'fan-EA'} {g['height']}^2 {g['views']}x{g['dets']} B={B}"] = rows
+        tag = f"{'fan-ED' if cfg == 1 else 'fan-EA'} {g['height']}^2 {g['views']}x{g['dets']} B={B}"
+        out["configs"][tag + ("" if dt == torch.float32 else " fp64")] = rows
     return out
 
 
