@@ -63,14 +63,22 @@ __device__ __forceinline__ void lap_offer(T (&lv)[KMAX], int (&li)[KMAX], T v, i
     }
 }
 
+constexpr int LAP_SB = 16;  // candidates per deferred-insertion batch
+
 // Block (bx, by): queries bx*256 .. +255 against candidate slice by (tiles
 // [by*tps, (by+1)*tps) of 64 points).  Writes the slice's sorted KMAX-list of
-// every query to part[by][q][0..KMAX).
+// every query to part[by][q][0..KMAX).  Insertions are deferred: per batch of
+// 16 candidates each thread appends the ones below its current k-th entry to
+// its own LDS column, then the wave inserts while any lane has one left, so a
+// wave runs as many insertions as its busiest lane accepted instead of one for
+// every candidate that some lane accepted.
 template <typename T, int DMAX, int KMAX>
 __global__ __launch_bounds__(256) void k_lap_knn(const T* __restrict__ x, const T* __restrict__ norm, int n, int d,
                                                  int tps, T* __restrict__ pv, int* __restrict__ pc) {
     __shared__ T ct[LAP_TC][DMAX];
     __shared__ T cn[LAP_TC];
+    __shared__ T bv[LAP_SB][LAP_NT];  // accepted (value, column) per thread
+    __shared__ int bc[LAP_SB][LAP_NT];
     const int tid = threadIdx.x;
     const int q = blockIdx.x * LAP_NT + tid;
     const bool active = q < n;
@@ -97,11 +105,24 @@ __global__ __launch_bounds__(256) void k_lap_knn(const T* __restrict__ x, const 
         if (tid < LAP_TC) cn[tid] = tid < nc ? norm[c0 + tid] : T(0);
         __syncthreads();
         if (!active) continue;
-        for (int ci = 0; ci < nc; ci++) {
-            T dot = 0;
+        for (int cb = 0; cb < nc; cb += LAP_SB) {
+            const int ce = nc < cb + LAP_SB ? nc : cb + LAP_SB;
+            const T tv = lv[KMAX - 1];
+            const int tc = li[KMAX - 1];
+            int cnt = 0;
+            for (int ci = cb; ci < ce; ci++) {
+                T dot = 0;
 #pragma unroll
-            for (int j = 0; j < DMAX; j++) dot = fma(qv[j], ct[ci][j], dot);
-            lap_offer<T, KMAX>(lv, li, fma(T(-2), dot, nq + cn[ci]), c0 + ci);
+                for (int j = 0; j < DMAX; j++) dot = fma(qv[j], ct[ci][j], dot);
+                const T v = fma(T(-2), dot, nq + cn[ci]);
+                if (lap_before(v, c0 + ci, tv, tc)) {
+                    bv[cnt][tid] = v;
+                    bc[cnt][tid] = c0 + ci;
+                    cnt++;
+                }
+            }
+            for (int i = 0; __ballot(i < cnt) != 0; i++)
+                if (i < cnt) lap_offer<T, KMAX>(lv, li, bv[i][tid], bc[i][tid]);
         }
     }
     if (!active) return;
