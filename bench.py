@@ -201,8 +201,8 @@ def workload_ops(args):
                 xin = (x[:1, 0] if name in img_in else y[:1, 0, :vs]).double().cpu().numpy()
                 t0 = time.perf_counter()
                 O.apply(name, o_sub, xin)
-                dt = time.perf_counter() - t0
-                row["cpu_oracle_calls_per_s"] = round(1.0 / (dt * (g["views"] / vs) * B), 5)
+                t_cpu = time.perf_counter() - t0
+                row["cpu_oracle_calls_per_s"] = round(1.0 / (t_cpu * (g["views"] / vs) * B), 5)
                 row["cpu_sample"] = f"1 slice x {vs} views, {O.max_threads()} threads, scaled"
             rows[name] = row
         tag = f"{'fan-ED' if cfg == 1 else 'fan-EA'} {g['height']}^2 {g['views']}x{g['dets']} B={B}"
