@@ -383,8 +383,9 @@ def main():
                          "ops: every public operator; laplacian: kNN graph weights")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; the projector shards over slices / views "
-                                   "(see the design notes) but no multi-GPU path is measured", "n_gpus": args.gpus}))
+        print(json.dumps({"error": "this build runs on one GPU only: the projector would shard over slices "
+                                   "(no exchange) or views (one image all-reduce per adjoint), but no multi-GPU "
+                                   "path is built or measured", "n_gpus": args.gpus}))
         sys.exit(2)
     if args.calib_copy:
         calib_copy()

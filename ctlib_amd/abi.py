@@ -1,6 +1,6 @@
 """ctypes binding of the C ABI in libctlib_amd.so (include/ctlib_abi.h).
 
-This is the binding a non-torch caller would use; the
+This is the binding a non-torch caller would use (a ctypes stub over the C ABI); the
 package's torch ops go through the C++ extension `ctlib_amd._C` instead.  It
 is also what the host-side tests use to check that the library loads, exports
 every declared symbol, and builds the same geometry plan on any machine.
