@@ -17,19 +17,38 @@ namespace {
 
 // One launch covers both sweep axes: work items [0, B*ntx) are (b, tile of
 // 4*LPW image rows) over the X-axis views, the rest (b, tile of columns) over
-// the Y-axis views, written bottom-to-top to outYT.
+// the Y-axis views, written bottom-to-top to outYT.  With nch > 1 every item is
+// split into nch chunks of its axis's views; chunk k writes its partial image
+// k*B*H*W elements into outX / outYT.
 template <typename T, int LPW, int NPE, bool WEIGHTED, int GEO, bool EXACT>
 __global__ __launch_bounds__(256) void k_adj(DevPlan P, const double2* __restrict__ tabs, T* __restrict__ outX,
-                                             T* __restrict__ outYT, int B, int ntx, int nty, double wscale) {
+                                             T* __restrict__ outYT, int B, int ntx, int nty, double wscale, int nch) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int nwx = B * ntx;
-    int w = xcd_remap(blockIdx.x, gridDim.x, nwx + B * nty);
+    int w = xcd_remap(blockIdx.x, gridDim.x, (nwx + B * nty) * nch);
     if (w < 0) return;
+    const int ch = w % nch;
+    w /= nch;
     const int mode = w < nwx ? MODE_X : MODE_Y;
     if (mode == MODE_Y) w -= nwx;
     const int ntiles = mode == MODE_Y ? nty : ntx;
     const int b = w / ntiles, tile = w - b * ntiles;
-    adj_block<T, LPW, NPE, WEIGHTED, GEO, EXACT>(P, tabs, outX, outYT, b, mode, tile, wscale, smem);
+    const int nv = mode == MODE_Y ? P.ny : P.nx;
+    const size_t part = (size_t)ch * B * P.H * P.W;
+    adj_block<T, LPW, NPE, WEIGHTED, GEO, EXACT>(P, tabs, outX + part, outYT + part, b, mode, tile, wscale, smem,
+                                                 ch * nv / nch, (ch + 1) * nv / nch);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_combine_parts(T* __restrict__ out, const T* __restrict__ xs,
+                                                       const T* __restrict__ ys, int nch, int H, int W, int hasX,
+                                                       int hasY) {
+    __shared__ T tile[32][33];
+    const int b = blockIdx.z;
+    const int B = gridDim.z;
+    const size_t img = (size_t)H * W;
+    combine_parts_tile<T>(out + b * img, xs + b * img, ys + b * img, nch, (size_t)B * img, H, W, blockIdx.y * 32,
+                          blockIdx.x * 32, hasX, hasY, tile);
 }
 
 // backprojection_sv: one view's term of the weighted backprojection, written to
@@ -119,11 +138,11 @@ namespace {
 
 template <typename T, int LPW, int NPE, bool WTD, int GEO, bool EXACT>
 hipError_t adj_launch_t(const DevPlan& P, const double2* tabs, void* outX, void* outYT, int B, double wscale,
-                        hipStream_t s) {
+                        int nch, hipStream_t s) {
     const int L = 4 * LPW;
     const int ntx = P.nx ? (P.H + L - 1) / L : 0;
     const int nty = P.ny ? (P.W + L - 1) / L : 0;
-    const int nwork = (ntx + nty) * B;
+    const int nwork = (ntx + nty) * B * nch;
     const size_t lds = adj_block_lds(P.D, L);
     auto kern = k_adj<T, LPW, NPE, WTD, GEO, EXACT>;
     if (lds > 64 * 1024) {
@@ -131,55 +150,77 @@ hipError_t adj_launch_t(const DevPlan& P, const double2* tabs, void* outX, void*
         if (e != hipSuccess) return e;
     }
     hipLaunchKernelGGL(kern, dim3(grid_for(nwork)), dim3(NT), lds, s, P, tabs, (T*)outX, (T*)outYT, B, ntx, nty,
-                       wscale);
+                       wscale, nch);
     return hipGetLastError();
 }
 
 template <typename T, bool WTD, int GEO>
-hipError_t adj_npe(const DevPlan& P, const double2* tabs, void* outX, void* outYT, int B, double wscale,
+hipError_t adj_npe(const DevPlan& P, const double2* tabs, void* outX, void* outYT, int B, double wscale, int nch,
                    hipStream_t s) {
     int n = P.nx ? P.W : 0;
     if (P.ny && P.H > n) n = P.H;
     const int npe = (n + 1 + 63) / 64;
     // every line (rows of X views, columns of Y views) exactly 64 * (NPE - 1) pixels
     auto exact = [&](int NPE) { return (P.nx == 0 || P.W == 64 * (NPE - 1)) && (P.ny == 0 || P.H == 64 * (NPE - 1)); };
-    if (npe <= 5) return exact(5) ? adj_launch_t<T, 8, 5, WTD, GEO, true>(P, tabs, outX, outYT, B, wscale, s)
-                                  : adj_launch_t<T, 8, 5, WTD, GEO, false>(P, tabs, outX, outYT, B, wscale, s);
-    if (npe <= 7) return exact(7) ? adj_launch_t<T, 4, 7, WTD, GEO, true>(P, tabs, outX, outYT, B, wscale, s)
-                                  : adj_launch_t<T, 4, 7, WTD, GEO, false>(P, tabs, outX, outYT, B, wscale, s);
-    if (npe <= 9) return exact(9) ? adj_launch_t<T, 4, 9, WTD, GEO, true>(P, tabs, outX, outYT, B, wscale, s)
-                                  : adj_launch_t<T, 4, 9, WTD, GEO, false>(P, tabs, outX, outYT, B, wscale, s);
-    if (npe <= 13) return exact(13) ? adj_launch_t<T, 2, 13, WTD, GEO, true>(P, tabs, outX, outYT, B, wscale, s)
-                                    : adj_launch_t<T, 2, 13, WTD, GEO, false>(P, tabs, outX, outYT, B, wscale, s);
-    if (npe <= 17) return exact(17) ? adj_launch_t<T, 2, 17, WTD, GEO, true>(P, tabs, outX, outYT, B, wscale, s)
-                                    : adj_launch_t<T, 2, 17, WTD, GEO, false>(P, tabs, outX, outYT, B, wscale, s);
-    if (npe <= 33) return adj_launch_t<T, 1, 33, WTD, GEO, false>(P, tabs, outX, outYT, B, wscale, s);
+    if (npe <= 5) return exact(5) ? adj_launch_t<T, 8, 5, WTD, GEO, true>(P, tabs, outX, outYT, B, wscale, nch, s)
+                                  : adj_launch_t<T, 8, 5, WTD, GEO, false>(P, tabs, outX, outYT, B, wscale, nch, s);
+    if (npe <= 7) return exact(7) ? adj_launch_t<T, 4, 7, WTD, GEO, true>(P, tabs, outX, outYT, B, wscale, nch, s)
+                                  : adj_launch_t<T, 4, 7, WTD, GEO, false>(P, tabs, outX, outYT, B, wscale, nch, s);
+    if (npe <= 9) return exact(9) ? adj_launch_t<T, 4, 9, WTD, GEO, true>(P, tabs, outX, outYT, B, wscale, nch, s)
+                                  : adj_launch_t<T, 4, 9, WTD, GEO, false>(P, tabs, outX, outYT, B, wscale, nch, s);
+    if (npe <= 13) return exact(13) ? adj_launch_t<T, 2, 13, WTD, GEO, true>(P, tabs, outX, outYT, B, wscale, nch, s)
+                                    : adj_launch_t<T, 2, 13, WTD, GEO, false>(P, tabs, outX, outYT, B, wscale, nch, s);
+    if (npe <= 17) return exact(17) ? adj_launch_t<T, 2, 17, WTD, GEO, true>(P, tabs, outX, outYT, B, wscale, nch, s)
+                                    : adj_launch_t<T, 2, 17, WTD, GEO, false>(P, tabs, outX, outYT, B, wscale, nch, s);
+    if (npe <= 33) return adj_launch_t<T, 1, 33, WTD, GEO, false>(P, tabs, outX, outYT, B, wscale, nch, s);
     return hipErrorInvalidValue;
 }
 
 template <typename T, bool WTD>
-hipError_t adj_geo(const DevPlan& P, const double2* tabs, void* outX, void* outYT, int B, double wscale,
+hipError_t adj_geo(const DevPlan& P, const double2* tabs, void* outX, void* outYT, int B, double wscale, int nch,
                    hipStream_t s) {
     switch (P.scan_type) {
-    case GEO_ED: return adj_npe<T, WTD, GEO_ED>(P, tabs, outX, outYT, B, wscale, s);
+    case GEO_ED: return adj_npe<T, WTD, GEO_ED>(P, tabs, outX, outYT, B, wscale, nch, s);
     case GEO_EA:
-        return P.ea_poly ? adj_npe<T, WTD, GEO_EAP>(P, tabs, outX, outYT, B, wscale, s)
-                         : adj_npe<T, WTD, GEO_EA>(P, tabs, outX, outYT, B, wscale, s);
-    default: return adj_npe<T, WTD, GEO_PARA>(P, tabs, outX, outYT, B, wscale, s);
+        return P.ea_poly ? adj_npe<T, WTD, GEO_EAP>(P, tabs, outX, outYT, B, wscale, nch, s)
+                         : adj_npe<T, WTD, GEO_EA>(P, tabs, outX, outYT, B, wscale, nch, s);
+    default: return adj_npe<T, WTD, GEO_PARA>(P, tabs, outX, outYT, B, wscale, nch, s);
     }
 }
 
 }  // namespace
 
+int adj_tile_lines(const DevPlan& P) {
+    int n = P.nx ? P.W : 0;
+    if (P.ny && P.H > n) n = P.H;
+    const int npe = (n + 1 + 63) / 64;
+    return npe <= 5 ? 32 : npe <= 9 ? 16 : npe <= 17 ? 8 : 4;
+}
+
+hipError_t launch_combine_parts(void* out, const void* xs, const void* ys, int dtype, int B, int H, int W, int nch,
+                                bool hasX, bool hasY, hipStream_t s) {
+    if (B <= 0) return hipSuccess;
+    dim3 grid((W + 31) / 32, (H + 31) / 32, B);
+    if (dtype == DT_F32)
+        hipLaunchKernelGGL(k_combine_parts<float>, grid, dim3(256), 0, s, (float*)out, (const float*)xs,
+                           (const float*)ys, nch, H, W, (int)hasX, (int)hasY);
+    else
+        hipLaunchKernelGGL(k_combine_parts<double>, grid, dim3(256), 0, s, (double*)out, (const double*)xs,
+                           (const double*)ys, nch, H, W, (int)hasX, (int)hasY);
+    return hipGetLastError();
+}
+
 hipError_t launch_adj(const DevPlan& P, int dtype, const double2* tabs, void* outX, void* outYT, int B, bool weighted,
-                      double wscale, AdjShape, hipStream_t s) {
+                      double wscale, AdjShape sh, hipStream_t s) {
     if (B <= 0 || P.nx + P.ny == 0) return hipSuccess;
+    if (sh.nch < 1) sh.nch = 1;
+    if ((P.nx && sh.nch > P.nx) || (P.ny && sh.nch > P.ny)) return hipErrorInvalidValue;
     if (dtype == DT_F32) {
-        return weighted ? adj_geo<float, true>(P, tabs, outX, outYT, B, wscale, s)
-                        : adj_geo<float, false>(P, tabs, outX, outYT, B, wscale, s);
+        return weighted ? adj_geo<float, true>(P, tabs, outX, outYT, B, wscale, sh.nch, s)
+                        : adj_geo<float, false>(P, tabs, outX, outYT, B, wscale, sh.nch, s);
     }
-    return weighted ? adj_geo<double, true>(P, tabs, outX, outYT, B, wscale, s)
-                    : adj_geo<double, false>(P, tabs, outX, outYT, B, wscale, s);
+    return weighted ? adj_geo<double, true>(P, tabs, outX, outYT, B, wscale, sh.nch, s)
+                    : adj_geo<double, false>(P, tabs, outX, outYT, B, wscale, sh.nch, s);
 }
 
 namespace {

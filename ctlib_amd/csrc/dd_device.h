@@ -151,6 +151,39 @@ __device__ __forceinline__ void combine_tile(T* __restrict__ ob, const T* __rest
     }
 }
 
+// out[r][c] = sum_k xs[k][r][c] (hasX) + sum_k ys[k][c][H-1-r] (hasY) over the nch
+// partial images of a view-chunked bin -> pixel launch, for the tile at (r0, c0);
+// partial k of the X (Y) images starts k*pstride elements further.
+template <typename T>
+__device__ __forceinline__ void combine_parts_tile(T* __restrict__ ob, const T* __restrict__ xs,
+                                                   const T* __restrict__ ys, int nch, size_t pstride, int H, int W,
+                                                   int r0, int c0, int hasX, int hasY, T (*tile)[33]) {
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    if (hasY) {
+#pragma unroll
+        for (int k = 0; k < 32; k += 8) {
+            const int c = c0 + ty + k;
+            const int r = r0 + 31 - tx;
+            T v = T(0);
+            if (c < W && r < H)
+                for (int ch = 0; ch < nch; ch++) v += ys[ch * pstride + (size_t)c * H + (H - 1 - r)];
+            tile[ty + k][31 - tx] = v;
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int k = 0; k < 32; k += 8) {
+        const int r = r0 + ty + k, c = c0 + tx;
+        if (r < H && c < W) {
+            T v = T(0);
+            if (hasX)
+                for (int ch = 0; ch < nch; ch++) v += xs[ch * pstride + (size_t)r * W + c];
+            if (hasY) v += tile[tx][ty + k];
+            ob[(size_t)r * W + c] = v;
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Line -> bin (projection / backprojection_t)
 // ---------------------------------------------------------------------------
@@ -292,11 +325,13 @@ __host__ __device__ inline size_t fwd_block_lds(int nmax, int G, bool weighted, 
 // EDGEX (unweighted): D == 64*NEM, so the last bin edge D is the only edge of
 // group NEM; lane v of the wave evaluates it for view v (VPW <= 64), one
 // evaluation per line for all the wave's views instead of a mostly idle group.
+// Only lines [lb, le) of the axis contribute (le < 0: all): a launch split into
+// line chunks writes each chunk's partial sinogram and sums them afterwards.
 template <typename T, int VPW, int NEM, int M, bool WEIGHTED, bool RESID, bool FULL = false, bool EDGEX = false>
 __device__ __forceinline__ void fwd_block(const DevPlan& P, const T* __restrict__ src, T* __restrict__ y, int b,
                                           int mode, int grp, const double* __restrict__ post,
                                           const T* __restrict__ m, double2* __restrict__ tabs,
-                                          unsigned char* __restrict__ smem) {
+                                          unsigned char* __restrict__ smem, int lb = 0, int le = -1) {
     constexpr int G = 4 * VPW;
     constexpr int NTAB = WEIGHTED ? G : 1;
     const int tid = threadIdx.x, lane = tid & 63;
@@ -304,6 +339,7 @@ __device__ __forceinline__ void fwd_block(const DevPlan& P, const T* __restrict_
     const int* vlist = mode == MODE_Y ? P.yviews : P.xviews;
     const int nv = mode == MODE_Y ? P.ny : P.nx;
     const int nlines = mode == MODE_Y ? P.W : P.H;
+    const int lend = le < 0 ? nlines : le;
     const int n = FULL ? NT * M : (mode == MODE_Y ? P.H : P.W);
     const int D = P.D;
 
@@ -349,13 +385,15 @@ __device__ __forceinline__ void fwd_block(const DevPlan& P, const T* __restrict_
         const int idx = grp * G + cg;
         crow = reinterpret_cast<const double*>(P.lines + (size_t)vlist[idx < nv ? idx : nv - 1] * P.max_lines) + cw;
     }
-    double cnext = cthread ? crow[0] : 0.0;
+    const bool any = lb < lend;
+    double cnext = cthread && any ? crow[(size_t)lb * LC_WORDS] : 0.0;
 
     T xv[M];
+    const T* lb0 = base + (size_t)lb * n;
 #pragma unroll
-    for (int i = 0; i < M; i++) xv[i] = (FULL || (i < m_px && p0 + i < n)) ? base[p0 + i] : T(0);
+    for (int i = 0; i < M; i++) xv[i] = any && (FULL || (i < m_px && p0 + i < n)) ? lb0[p0 + i] : T(0);
 
-    for (int l = 0; l < nlines; l++) {
+    for (int l = lb; l < lend; l++) {
         LEnt<T>* tb = tab + (l & 1) * NTAB * tabn;
         double* ws = wsum + (l & 1) * NTAB * 4;
         double* cfl = cf + (l & 1) * G * LC_WORDS;
@@ -383,7 +421,7 @@ __device__ __forceinline__ void fwd_block(const DevPlan& P, const T* __restrict_
             }
         }
         line_tables<T, M, NTAB, FULL>(val, m_px, p0, n, tb, tabn, ws, tid, lane, wave);
-        if (l + 1 < nlines) {
+        if (l + 1 < lend) {
             const T* nb = base + (size_t)(l + 1) * n;
 #pragma unroll
             for (int i = 0; i < M; i++) xv[i] = (FULL || (i < m_px && p0 + i < n)) ? nb[p0 + i] : T(0);
@@ -440,10 +478,11 @@ __host__ __device__ inline size_t fwd_w_block_lds(int nmax) { return (size_t)4 *
 // arithmetic with a fp32 reciprocal, about 1e-7 relative; fp64 data: fp64
 // arithmetic with a fp32 reciprocal), and the weighted value is rounded to the
 // data type before the fp64 prefix sum, as in fwd_block with WEIGHTED.
+// Lines [lb, le) of the axis contribute (le < 0: all), as in fwd_block.
 template <typename T, int NEM, int PPL, bool FULL>
 __device__ __forceinline__ void fwd_block_w(const DevPlan& P, const T* __restrict__ src, T* __restrict__ y, int b,
                                             int mode, int grp, const double* __restrict__ post,
-                                            unsigned char* __restrict__ smem) {
+                                            unsigned char* __restrict__ smem, int lb = 0, int le = -1) {
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int* vlist = mode == MODE_Y ? P.yviews : P.xviews;
@@ -477,11 +516,14 @@ __device__ __forceinline__ void fwd_block_w(const DevPlan& P, const T* __restric
         for (int k = 0; k < PPL; k++) xv[k] = (FULL || p0 + k < n) ? row[p0 + k] : T(0);
         cw = crow[(size_t)l * LC_WORDS];
     };
+    const int lend = le < 0 ? nlines : le;
     T xv[PPL];
-    double cw;
-    load_line(0, xv, cw);
+    double cw = 0.0;
+#pragma unroll
+    for (int k = 0; k < PPL; k++) xv[k] = T(0);
+    if (lb < lend) load_line(lb, xv, cw);
 
-    for (int l = 0; l < nlines; l++) {
+    for (int l = lb; l < lend; l++) {
         const double s = readlane_d(cw, LC_S), A = readlane_d(cw, LC_A), Bc = readlane_d(cw, LC_B);
         const double w0 = readlane_d(cw, LC_W0), w1 = readlane_d(cw, LC_W1);
         const double w2 = readlane_d(cw, LC_W2), is = readlane_d(cw, LC_INVS);
@@ -507,7 +549,7 @@ __device__ __forceinline__ void fwd_block_w(const DevPlan& P, const T* __restric
         double run = 0.0;
 #pragma unroll
         for (int k = 0; k < PPL; k++) run += val[k];
-        if (l + 1 < nlines) load_line(l + 1, xv, cw);
+        if (l + 1 < lend) load_line(l + 1, xv, cw);
         // Table of this line: the wave's reads of the previous line's table were
         // consumed above (in-order LDS within a wave), so it can be overwritten.
         const double inc = wave_scan_incl(run, lane);
@@ -598,16 +640,18 @@ __host__ __device__ inline size_t adj_block_lds(int D, int L) {
 // the only edge of group NPE-1; instead of a ninth (or seventeenth) group per
 // line, lane q of the wave evaluates edge n of its line q (LPW <= 64 lanes),
 // one evaluation per view for all the wave's lines.
+// Views [vb, ve) of the axis list (ve < 0: all) contribute; a launch split into
+// view chunks writes each chunk's partial image and sums them afterwards.
 template <typename T, int LPW, int NPE, bool WEIGHTED, int GEO, bool EXACT = false>
 __device__ __forceinline__ void adj_block(const DevPlan& P, const double2* __restrict__ tabs, T* __restrict__ outX,
                                           T* __restrict__ outYT, int b, int mode, int tile, double wscale,
-                                          unsigned char* __restrict__ smem) {
+                                          unsigned char* __restrict__ smem, int vb = 0, int ve = -1) {
     constexpr int L = 4 * LPW;
     constexpr int NG = EXACT ? NPE - 1 : NPE;  // edge groups evaluated per line
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int* vlist = mode == MODE_Y ? P.yviews : P.xviews;
-    const int nv = mode == MODE_Y ? P.ny : P.nx;
+    const int nv = ve < 0 ? (mode == MODE_Y ? P.ny : P.nx) : ve;
     const int nlines = mode == MODE_Y ? P.W : P.H;
     const int n = mode == MODE_Y ? P.H : P.W;
     const int D = P.D;
@@ -646,13 +690,13 @@ __device__ __forceinline__ void adj_block(const DevPlan& P, const double2* __res
     double accx = 0.0;                          // EXACT: lane q accumulates edge n of line q
     const int lx = (lane < LPW ? lane : 0) * 4 + wave;  // that line within the tile
 
-    table_to_lds(vlist[0], 0);
-    if (cthread) scf[tid] = coef_word(vlist[0]);
+    table_to_lds(vlist[vb], 0);
+    if (cthread) scf[tid] = coef_word(vlist[vb]);
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     __syncthreads();
 
-    for (int iv = 0; iv < nv; iv++) {
-        const int cur = iv & 1;
+    for (int iv = vb; iv < nv; iv++) {
+        const int cur = (iv - vb) & 1;
         const int v = vlist[iv];
         double cnext = 0.0;
         if (iv + 1 < nv) {

@@ -22,16 +22,20 @@ constexpr int fwd_w_waves() { return sizeof(T) == 4 && FULL && PPL == 8 && NEM <
 template <typename T, int NEM, int PPL, bool FULL>
 __global__ __launch_bounds__(256, (fwd_w_waves<T, NEM, PPL, FULL>())) void k_fwd_w(DevPlan P, const T* __restrict__ srcx, const T* __restrict__ srcy,
                                                T* __restrict__ y, int B, int ngx, int ngy,
-                                               const double* __restrict__ post) {
+                                               const double* __restrict__ post, int nch) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int nwx = B * ngx;
-    int w = xcd_remap(blockIdx.x, gridDim.x, nwx + B * ngy);
+    int w = xcd_remap(blockIdx.x, gridDim.x, (nwx + B * ngy) * nch);
     if (w < 0) return;
+    const int ch = w % nch;
+    w /= nch;
     const int mode = w < nwx ? MODE_X : MODE_Y;
     if (mode == MODE_Y) w -= nwx;
     const int ngroups = mode == MODE_Y ? ngy : ngx;
     const int b = w / ngroups, grp = w - b * ngroups;
-    fwd_block_w<T, NEM, PPL, FULL>(P, mode == MODE_Y ? srcy : srcx, y, b, mode, grp, post, smem);
+    const int nl = mode == MODE_Y ? P.W : P.H;
+    fwd_block_w<T, NEM, PPL, FULL>(P, mode == MODE_Y ? srcy : srcx, y + (size_t)ch * B * P.V * P.D, b, mode, grp,
+                                   post, smem, ch * nl / nch, (ch + 1) * nl / nch);
 }
 
 int line_max(const DevPlan& P) {
@@ -46,12 +50,13 @@ struct Args {
     void* y;
     int B;
     const double* post;
+    int nch;
 };
 
 template <typename T, int NEM, int PPL, bool FULL>
 hipError_t launch_t(const DevPlan& P, const Args& a, hipStream_t s) {
     const int ngx = (P.nx + 3) / 4, ngy = (P.ny + 3) / 4;
-    const int nwork = (ngx + ngy) * a.B;
+    const int nwork = (ngx + ngy) * a.B * a.nch;
     const size_t lds = fwd_w_block_lds(line_max(P));
     auto kern = k_fwd_w<T, NEM, PPL, FULL>;
     if (lds > 64 * 1024) {
@@ -59,7 +64,7 @@ hipError_t launch_t(const DevPlan& P, const Args& a, hipStream_t s) {
         if (e != hipSuccess) return e;
     }
     hipLaunchKernelGGL(kern, dim3(grid_for(nwork)), dim3(NT), lds, s, P, (const T*)a.srcx, (const T*)a.srcy, (T*)a.y,
-                       a.B, ngx, ngy, a.post);
+                       a.B, ngx, ngy, a.post, a.nch);
     return hipGetLastError();
 }
 
@@ -102,10 +107,12 @@ hipError_t by_nem(const DevPlan& P, const Args& a, hipStream_t s) {
 bool fwd_w_supported(const DevPlan& P) { return line_max(P) <= 1024 && P.D <= 2046; }
 
 hipError_t launch_fwd_w(const DevPlan& P, int dtype, const void* srcx, const void* srcy, void* y, int B,
-                        const double* post, hipStream_t s) {
+                        const double* post, int nch, hipStream_t s) {
     if (B <= 0 || P.nx + P.ny == 0) return hipSuccess;
     if (!fwd_w_supported(P)) return hipErrorInvalidValue;
-    Args a{srcx, srcy, y, B, post};
+    if (nch < 1) nch = 1;
+    if ((P.nx && nch > P.H) || (P.ny && nch > P.W)) return hipErrorInvalidValue;
+    Args a{srcx, srcy, y, B, post, nch};
     if (dtype == DT_F32) return by_nem<float>(P, a, s);
     return by_nem<double>(P, a, s);
 }

@@ -37,17 +37,32 @@ template <typename T, int VPW, int NEM, int M, bool WEIGHTED, bool RESID, bool F
 __global__ __launch_bounds__(256, (fwd_waves<T, NEM, M, WEIGHTED>())) void k_fwd(DevPlan P, const T* __restrict__ srcx, const T* __restrict__ srcy,
                                              T* __restrict__ y, int B, int ngx, int ngy,
                                              const double* __restrict__ post, const T* __restrict__ m,
-                                             double2* __restrict__ tabs) {
+                                             double2* __restrict__ tabs, int nch) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int nwx = B * ngx;
-    int w = xcd_remap(blockIdx.x, gridDim.x, nwx + B * ngy);
+    int w = xcd_remap(blockIdx.x, gridDim.x, (nwx + B * ngy) * nch);
     if (w < 0) return;
+    const int ch = w % nch;
+    w /= nch;
     const int mode = w < nwx ? MODE_X : MODE_Y;
     if (mode == MODE_Y) w -= nwx;
     const int ngroups = mode == MODE_Y ? ngy : ngx;
     const int b = w / ngroups, grp = w - b * ngroups;
-    fwd_block<T, VPW, NEM, M, WEIGHTED, RESID, FULL, EDGEX>(P, mode == MODE_Y ? srcy : srcx, y, b, mode, grp, post, m,
-                                                            tabs, smem);
+    const int nl = mode == MODE_Y ? P.W : P.H;
+    fwd_block<T, VPW, NEM, M, WEIGHTED, RESID, FULL, EDGEX>(P, mode == MODE_Y ? srcy : srcx,
+                                                            y + (size_t)ch * B * P.V * P.D, b, mode, grp, post, m,
+                                                            tabs, smem, ch * nl / nch, (ch + 1) * nl / nch);
+}
+
+// out[i] = sum_k parts[k * count + i], k < nch (partial sinograms of a line-chunked launch)
+template <typename T>
+__global__ __launch_bounds__(256) void k_sum_parts(T* __restrict__ out, const T* __restrict__ parts, int nch,
+                                                   size_t count) {
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < count; i += (size_t)gridDim.x * 256) {
+        T v = parts[i];
+        for (int k = 1; k < nch; k++) v += parts[(size_t)k * count + i];
+        out[i] = v;
+    }
 }
 
 // Per-(b, view) bin tables {C_j, g_j} of g_k = ((y - m)[o(k)] * pre[k]) * scale, j = 0..D+1.
@@ -131,13 +146,14 @@ struct FwdArgs {
     const double* post;
     const void* m;
     double2* tabs;
+    int nch;
 };
 
 template <typename T, int VPW, int NEM, int M, bool WTD, bool RESID, bool FULL, bool EDGEX>
 hipError_t fwd_launch_t(const DevPlan& P, const FwdArgs& a, hipStream_t s) {
     constexpr int G = 4 * VPW;
     const int ngx = (P.nx + G - 1) / G, ngy = (P.ny + G - 1) / G;
-    const int nwork = (ngx + ngy) * a.B;
+    const int nwork = (ngx + ngy) * a.B * a.nch;
     const size_t lds = fwd_lds_bytes(P, WTD, G);
     auto kern = k_fwd<T, VPW, NEM, M, WTD, RESID, FULL, EDGEX>;
     if (lds > 64 * 1024) {
@@ -145,7 +161,7 @@ hipError_t fwd_launch_t(const DevPlan& P, const FwdArgs& a, hipStream_t s) {
         if (e != hipSuccess) return e;
     }
     hipLaunchKernelGGL(kern, dim3(grid_for(nwork)), dim3(NT), lds, s, P, (const T*)a.srcx, (const T*)a.srcy, (T*)a.y,
-                       a.B, ngx, ngy, a.post, (const T*)a.m, a.tabs);
+                       a.B, ngx, ngy, a.post, (const T*)a.m, a.tabs, a.nch);
     return hipGetLastError();
 }
 
@@ -200,8 +216,10 @@ hipError_t launch_fwd(const DevPlan& P, int dtype, const void* srcx, const void*
                       const double* post, FwdShape shape, hipStream_t s) {
     if (B <= 0 || P.nx + P.ny == 0) return hipSuccess;
     if ((P.nx && P.W > NT * 8) || (P.ny && P.H > NT * 8) || P.D > 2046) return hipErrorInvalidValue;
-    if (weighted && fwd_w_supported(P)) return launch_fwd_w(P, dtype, srcx, srcy, y, B, post, s);
-    FwdArgs a{srcx, srcy, y, B, post, nullptr, nullptr};
+    const int nch = shape.nch < 1 ? 1 : shape.nch;
+    if ((P.nx && nch > P.H) || (P.ny && nch > P.W)) return hipErrorInvalidValue;
+    if (weighted && fwd_w_supported(P)) return launch_fwd_w(P, dtype, srcx, srcy, y, B, post, nch, s);
+    FwdArgs a{srcx, srcy, y, B, post, nullptr, nullptr, nch};
     if (dtype == DT_F32) return fwd_dispatch<float>(P, a, weighted, false, shape.G, s);
     return fwd_dispatch<double>(P, a, weighted, false, shape.G, s);
 }
@@ -210,9 +228,22 @@ hipError_t launch_fwd_resid(const DevPlan& P, int dtype, const void* srcx, const
                             double2* tabs, int B, FwdShape shape, hipStream_t s) {
     if (B <= 0 || P.nx + P.ny == 0) return hipSuccess;
     if ((P.nx && P.W > NT * 8) || (P.ny && P.H > NT * 8) || P.D > 2046) return hipErrorInvalidValue;
-    FwdArgs a{srcx, srcy, y, B, P.post_prj, m, tabs};
+    FwdArgs a{srcx, srcy, y, B, P.post_prj, m, tabs, 1};
     if (dtype == DT_F32) return fwd_dispatch<float>(P, a, false, true, shape.G, s);
     return fwd_dispatch<double>(P, a, false, true, shape.G, s);
+}
+
+hipError_t launch_sum_parts(void* out, const void* parts, int dtype, int nch, size_t count, hipStream_t s) {
+    if (count == 0) return hipSuccess;
+    size_t blocks = (count + 255) / 256;
+    if (blocks > 8192) blocks = 8192;
+    if (dtype == DT_F32)
+        hipLaunchKernelGGL(k_sum_parts<float>, dim3((unsigned)blocks), dim3(256), 0, s, (float*)out,
+                           (const float*)parts, nch, count);
+    else
+        hipLaunchKernelGGL(k_sum_parts<double>, dim3((unsigned)blocks), dim3(256), 0, s, (double*)out,
+                           (const double*)parts, nch, count);
+    return hipGetLastError();
 }
 
 hipError_t launch_bin_tables(const DevPlan& P, int in_dtype, const void* y, const void* m, const double* pre,

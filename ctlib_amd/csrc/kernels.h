@@ -33,16 +33,24 @@ enum { DT_F32 = 0, DT_F64 = 1 };
 enum { MODE_X = 0, MODE_Y = 1 };
 
 // Launch-shape choices the autotuner may change (see tune.cpp).
-struct FwdShape { int G; };       // views per block of the line->bin kernel
-struct AdjShape { int lpw; };     // lines per wave of the bin->pixel kernel (0 = default)
+struct FwdShape {                 // line->bin kernel
+    int G;                        // views per block
+    int nch;                      // line chunks (<= 1: none)
+};
+struct AdjShape { int nch; };     // view chunks of the bin->pixel kernel (<= 1: none)
 
 // x [B][H][W] -> xT [B][W][H] with xT[b][c][i] = x[b][H-1-i][c]  (Y-mode lines, bottom to top)
 hipError_t launch_to_ylines(const void* x, void* xT, int dtype, int B, int H, int W, hipStream_t s);
 
 // Line -> bin operator (projection; weighted = backprojection_t), both sweep axes in one launch.
 // `srcx` holds image rows ([B][H][W]), `srcy` the bottom-to-top column copy ([B][W][H]).
+// With shape.nch = k > 1 each axis's image lines are split into k chunks (k <= lines of each
+// axis) and chunk c writes its partial sinogram at y + c*B*V*D (elements); launch_sum_parts
+// adds them up.
 hipError_t launch_fwd(const DevPlan& P, int dtype, const void* srcx, const void* srcy, void* y, int B,
                       bool weighted, const double* post, FwdShape shape, hipStream_t s);
+// out[i] = sum_k parts[k*count + i], k < nch
+hipError_t launch_sum_parts(void* out, const void* parts, int dtype, int nch, size_t count, hipStream_t s);
 size_t fwd_lds_bytes(const DevPlan& P, bool weighted, int G);
 
 // projection fused with the residual bin tables of projection_t: writes y = A x (y may be null)
@@ -54,7 +62,7 @@ hipError_t launch_fwd_resid(const DevPlan& P, int dtype, const void* srcx, const
 // tables, for lines of at most 1024 pixels (fwd_w_supported); launch_fwd dispatches to it.
 bool fwd_w_supported(const DevPlan& P);
 hipError_t launch_fwd_w(const DevPlan& P, int dtype, const void* srcx, const void* srcy, void* y, int B,
-                        const double* post, hipStream_t s);
+                        const double* post, int nch, hipStream_t s);
 
 // Per (b, view) cumulative-integral tables {C_j, g_j}, j = 0..D+1, of g_k = (y - m)[o(k)] * pre[k] * scale.
 hipError_t launch_bin_tables(const DevPlan& P, int in_dtype, const void* y, const void* m, const double* pre,
@@ -62,8 +70,15 @@ hipError_t launch_bin_tables(const DevPlan& P, int in_dtype, const void* y, cons
 
 // Bin -> pixel operator (projection_t; weighted = backprojection, pixel weight * wscale), both axes in
 // one launch.  X lines are written to outX [B][H][W]; Y lines to outYT [B][W][H] (bottom-to-top per column).
+// With shape.nch = k > 1 each axis's views are split into k chunks (k <= views of each axis) and chunk c
+// writes its partial images at outX + c*B*H*W and outYT + c*B*H*W (elements); launch_combine_parts sums them.
 hipError_t launch_adj(const DevPlan& P, int dtype, const double2* tabs, void* outX, void* outYT, int B,
                       bool weighted, double wscale, AdjShape shape, hipStream_t s);
+// Image lines per work item of launch_adj (work items = B * (ceil(H/L) [X views] + ceil(W/L) [Y views])).
+int adj_tile_lines(const DevPlan& P);
+// out[b][r][c] = sum_k xs[k][b][r][c] (hasX) + sum_k ys[k][b][c][H-1-r] (hasY), k < nch
+hipError_t launch_combine_parts(void* out, const void* xs, const void* ys, int dtype, int B, int H, int W, int nch,
+                                bool hasX, bool hasY, hipStream_t s);
 
 // Per-view weighted backprojection (backprojection_sv): out[b][v][H][W] = view v's term of
 // backprojection, pixel weight * wscale, from the bin tables of launch_bin_tables.

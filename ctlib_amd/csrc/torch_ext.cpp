@@ -258,6 +258,12 @@ void set_plan_tuning(torch::Tensor options, int64_t device, int64_t fwd_views_pe
     if (rc) raise_status(rc, "set_plan_tuning");
 }
 
+void set_plan_chunks(torch::Tensor options, int64_t device, int64_t line_chunks, int64_t view_chunks) {
+    Cached c = get_plan(options, (int)device);
+    int rc = ctlib_plan_set_chunks(c.plan.get(), (int)line_chunks, (int)view_chunks);
+    if (rc) raise_status(rc, "set_plan_chunks");
+}
+
 void timing_enable(bool on) { ctlib_timing_enable(on ? 1 : 0); }
 
 // Per kernel class: launches, total milliseconds, algorithmic bytes and flops of the
@@ -393,6 +399,8 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m) {
         .def("check", &Batch::check)
         .def("info", &Batch::info);
     m.def("set_plan_tuning", &set_plan_tuning, "Override the line->bin views per block of a cached plan");
+    m.def("set_plan_chunks", &set_plan_chunks,
+          "Override the line->bin line chunks and bin->pixel view chunks of a cached plan");
     m.def("timing_enable", &timing_enable, "Record HIP events around the library's kernel launches");
     m.def("timing_collect", &timing_collect, "Per-class kernel times since the last collect");
     m.attr("OP_PRJ") = (int)CTLIB_OP_PRJ;

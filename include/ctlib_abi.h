@@ -100,6 +100,18 @@ size_t ctlib_residual_workspace_bytes(const ctlib_plan* plan, int64_t batch, int
  * used from several threads; it applies to later ctlib_run calls. */
 int ctlib_plan_set_tuning(ctlib_plan* plan, int fwd_views_per_block);
 
+/* Launch-shape overrides of work splitting, for batches too small to fill the
+ * GPU: the line->bin kernel (projection, backprojection_t, fbp_t) splits each
+ * sweep axis's image lines into `line_chunks` chunks, the bin->pixel kernel
+ * (projection_t, backprojection, fbp) its views into `view_chunks` chunks; the
+ * partial sinograms / images are summed afterwards.  0 = automatic (about 4096
+ * blocks per launch), 1 = no split, at most 64 (capped at the lines / views of
+ * each axis).  The workspace size depends on them: set them before
+ * ctlib_workspace_bytes and before the plan is used from several threads.
+ * ctlib_run_residual (forward fused with the residual tables) never splits
+ * lines.  No reference counterpart (launch shape only). */
+int ctlib_plan_set_chunks(ctlib_plan* plan, int line_chunks, int view_chunks);
+
 /* Geometry queries for host code that mirrors the plan (tests, autograd). */
 int ctlib_plan_geom(const ctlib_plan* plan, ctlib_geom* out);
 int ctlib_plan_view_info(const ctlib_plan* plan, int view, int* xmode, int* rev);

@@ -314,3 +314,44 @@ def test_forward_launch_shapes(dev, case, G):
     assert rel_l2(y[:, 0], y_ref) <= F32_TOL
     assert rel_l2(yr.cpu().numpy()[:, 0], y_ref) <= F32_TOL
     assert rel_l2(gr.cpu().numpy()[:, 0], g_ref) <= F32_TOL
+
+
+@pytest.mark.parametrize("nch", [0, 1, 2, 3, 7, 64])
+@pytest.mark.parametrize("case", ["ed_small", "ea_odd", "para_ties", "config1"])
+def test_line_and_view_chunks(dev, case, nch):
+    """Launches split into line chunks (line->bin) and view chunks (bin->pixel), with the
+    partial sinograms / images summed afterwards, give the oracle's result for every
+    operator and the residual step, at batch 1 and 2 (64 is capped at the lines / views
+    of the smaller sweep axis; 0 is the automatic choice)."""
+    from ctlib_amd import _C
+    if case == "config1":
+        o = phantom.CONFIGS[1]["options"]
+    elif case == "ed_small":
+        o = small_options(0, V=40, D=23, W=16, H=20)
+    elif case == "ea_odd":
+        o = small_options(1, V=33, D=31, W=13, H=11, Ang0=0.7)
+    else:
+        o = phantom.para_options(8, 19, 10, 10, 0.1, 0.13, Ang0=0.0, dAng=math.pi / 4, binshift=0.0)
+    rng = np.random.default_rng(23)
+    ot = torch.from_numpy(o)
+    o64 = o.astype(np.float64)
+    print(f"[case] line and view chunks {nch} {case}", flush=True)
+    for B in (1, 2):
+        ins = {op: inputs(op, o, B, rng) for op in OPS}
+        got = {}
+        try:
+            _C.set_plan_tuning(ot, dev.index or 0, 0)
+            _C.set_plan_chunks(ot, dev.index or 0, nch, nch)
+            for op in OPS:
+                got[op] = gpu_op(op, ins[op], o, dev)
+            x, y = ins["projection"], ins["projection_t"]
+            _, gr = ctlib_amd.residual_step(torch.from_numpy(x).to(dev), torch.from_numpy(y).to(dev), ot)
+            got["residual"] = gr.cpu().numpy()
+        finally:
+            _C.set_plan_chunks(ot, dev.index or 0, 0, 0)
+        for op in OPS:
+            ref = oracle_op(op, ins[op].astype(np.float64), o64)
+            assert rel_l2(got[op], ref) <= F32_TOL, (op, B, rel_l2(got[op], ref))
+        ax = O.apply("projection", o64, x[:, 0].astype(np.float64))
+        g_ref = O.apply("projection_t", o64, ax - y[:, 0].astype(np.float64))
+        assert rel_l2(got["residual"][:, 0], g_ref) <= F32_TOL, ("residual", B)
