@@ -710,10 +710,21 @@ int ctlib_run_residual(const ctlib_plan* p, int dtype, const void* x, const void
     const WsLayout la = layout(p, CTLIB_OP_PRJ, batch, dtype);
     const WsLayout lb = layout(p, CTLIB_OP_PRJ_T, batch, dtype);
     const size_t base = la.total > lb.total ? la.total : lb.total;
-    // forward fused with the residual tables, then the adjoint (no separate table pass)
     double2* tabs = reinterpret_cast<double2*>(w + base);
     void* yl = w + la.ylines;
     hipError_t e = hipSuccess;
+    if (la.fnch > 1) {
+        // small batch, forward split into line chunks: the partial sinograms are summed into y
+        // (into the first partial when y is not wanted), then the tables of (y - m) * post
+        // (a separate grid-wide sum measured faster than summing inside the per-row table pass)
+        void* yv = y ? y : w + la.sp;
+        e = run_to_sino(p, dt, x, yv, B, false, P.post_prj, la, w, s);
+        if (e == hipSuccess) e = run_tables(p, dt, yv, m, P.post_prj, tabs, B, s);
+        if (e == hipSuccess) e = run_to_image(p, dt, tabs, grad, B, false, 1.0, lb, w, s);
+        if (e != hipSuccess) return hip_fail(e, "kernel launch");
+        return CTLIB_OK;
+    }
+    // forward fused with the residual tables, then the adjoint (no separate table pass)
     if (P.ny) {
         KernelTimer t(K_AUX, 0, 0, s);
         e = launch_to_ylines(x, yl, dt, B, P.H, P.W, s);

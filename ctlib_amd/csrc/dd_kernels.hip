@@ -54,10 +54,10 @@ __global__ __launch_bounds__(256, (fwd_waves<T, NEM, M, WEIGHTED>())) void k_fwd
                                                             tabs, smem, ch * nl / nch, (ch + 1) * nl / nch);
 }
 
-// out[i] = sum_k parts[k * count + i], k < nch (partial sinograms of a line-chunked launch)
+// out[i] = sum_k parts[k * count + i], k < nch (partial sinograms of a line-chunked launch);
+// out may be the first partial (each element is read and written by the same thread).
 template <typename T>
-__global__ __launch_bounds__(256) void k_sum_parts(T* __restrict__ out, const T* __restrict__ parts, int nch,
-                                                   size_t count) {
+__global__ __launch_bounds__(256) void k_sum_parts(T* out, const T* parts, int nch, size_t count) {
     for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < count; i += (size_t)gridDim.x * 256) {
         T v = parts[i];
         for (int k = 1; k < nch; k++) v += parts[(size_t)k * count + i];

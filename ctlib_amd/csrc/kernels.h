@@ -49,7 +49,7 @@ hipError_t launch_to_ylines(const void* x, void* xT, int dtype, int B, int H, in
 // adds them up.
 hipError_t launch_fwd(const DevPlan& P, int dtype, const void* srcx, const void* srcy, void* y, int B,
                       bool weighted, const double* post, FwdShape shape, hipStream_t s);
-// out[i] = sum_k parts[k*count + i], k < nch
+// out[i] = sum_k parts[k*count + i], k < nch (out may equal parts)
 hipError_t launch_sum_parts(void* out, const void* parts, int dtype, int nch, size_t count, hipStream_t s);
 size_t fwd_lds_bytes(const DevPlan& P, bool weighted, int G);
 

@@ -108,8 +108,9 @@ int ctlib_plan_set_tuning(ctlib_plan* plan, int fwd_views_per_block);
  * blocks per launch), 1 = no split, at most 64 (capped at the lines / views of
  * each axis).  The workspace size depends on them: set them before
  * ctlib_workspace_bytes and before the plan is used from several threads.
- * ctlib_run_residual (forward fused with the residual tables) never splits
- * lines.  No reference counterpart (launch shape only). */
+ * ctlib_run_residual splits lines too (its forward is then not fused with the
+ * residual tables: y is summed from the partials and then y - m is tabulated).
+ * No reference counterpart (launch shape only). */
 int ctlib_plan_set_chunks(ctlib_plan* plan, int line_chunks, int view_chunks);
 
 /* Geometry queries for host code that mirrors the plan (tests, autograd). */
