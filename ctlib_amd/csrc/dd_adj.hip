@@ -142,7 +142,9 @@ hipError_t adj_launch_t(const DevPlan& P, const double2* tabs, void* outX, void*
     const int L = 4 * LPW;
     const int ntx = P.nx ? (P.H + L - 1) / L : 0;
     const int nty = P.ny ? (P.W + L - 1) / L : 0;
-    const int nwork = (ntx + nty) * B * nch;
+    const long long nw = (long long)(ntx + nty) * B * nch;
+    if (nw > (1LL << 30)) return hipErrorInvalidValue;  // grid size (one block per work item)
+    const int nwork = (int)nw;
     const size_t lds = adj_block_lds(P.D, L);
     auto kern = k_adj<T, LPW, NPE, WTD, GEO, EXACT>;
     if (lds > 64 * 1024) {

@@ -56,7 +56,9 @@ struct Args {
 template <typename T, int NEM, int PPL, bool FULL>
 hipError_t launch_t(const DevPlan& P, const Args& a, hipStream_t s) {
     const int ngx = (P.nx + 3) / 4, ngy = (P.ny + 3) / 4;
-    const int nwork = (ngx + ngy) * a.B * a.nch;
+    const long long nw = (long long)(ngx + ngy) * a.B * a.nch;
+    if (nw > (1LL << 30)) return hipErrorInvalidValue;  // grid size (one block per work item)
+    const int nwork = (int)nw;
     const size_t lds = fwd_w_block_lds(line_max(P));
     auto kern = k_fwd_w<T, NEM, PPL, FULL>;
     if (lds > 64 * 1024) {

@@ -153,7 +153,9 @@ template <typename T, int VPW, int NEM, int M, bool WTD, bool RESID, bool FULL, 
 hipError_t fwd_launch_t(const DevPlan& P, const FwdArgs& a, hipStream_t s) {
     constexpr int G = 4 * VPW;
     const int ngx = (P.nx + G - 1) / G, ngy = (P.ny + G - 1) / G;
-    const int nwork = (ngx + ngy) * a.B * a.nch;
+    const long long nw = (long long)(ngx + ngy) * a.B * a.nch;
+    if (nw > (1LL << 30)) return hipErrorInvalidValue;  // grid size (one block per work item)
+    const int nwork = (int)nw;
     const size_t lds = fwd_lds_bytes(P, WTD, G);
     auto kern = k_fwd<T, VPW, NEM, M, WTD, RESID, FULL, EDGEX>;
     if (lds > 64 * 1024) {
