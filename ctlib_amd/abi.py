@@ -20,7 +20,7 @@ OPS = {"projection": 0, "projection_t": 1, "backprojection": 2, "backprojection_
 
 # Every function declared in include/ctlib_abi.h.
 EXPORTS = ("ctlib_parse_options", "ctlib_plan_create", "ctlib_plan_destroy", "ctlib_workspace_bytes", "ctlib_run",
-           "ctlib_run_residual", "ctlib_residual_workspace_bytes", "ctlib_plan_geom", "ctlib_plan_view_info",
+           "ctlib_run_residual", "ctlib_run_residual_loss", "ctlib_residual_workspace_bytes", "ctlib_plan_geom", "ctlib_plan_view_info",
            "ctlib_plan_export", "ctlib_timing_enable", "ctlib_timing_collect", "ctlib_last_error",
            "ctlib_status_string", "ctlib_batch_create", "ctlib_batch_destroy", "ctlib_batch_workspace_bytes",
            "ctlib_batch_set_schedule", "ctlib_batch_bind", "ctlib_batch_run", "ctlib_batch_info",

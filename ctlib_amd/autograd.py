@@ -6,7 +6,7 @@ matched adjoint kernel as the hand-written backward.
     Backprojection   forward B          backward B^T   (exact adjoint, also for fan-EA)
     BackprojectionT  forward the reference's backprojection_t; backward its exact adjoint
     FBP              forward fbp        backward fbp_t
-    LeastSquares     0.5 * ||A x - m||^2 with A x and A^T (A x - m) from one fused call
+    LeastSquares     0.5 * ||A x - m||^2: the loss (per-row FP64 sums) and A^T (A x - m) from one call
 
 The reference ships no autograd layer; callers wrapped projection /
 projection_t themselves.  Note that for scan_type 1 the reference's
@@ -106,10 +106,10 @@ class LeastSquares(torch.autograd.Function):
 
     @staticmethod
     def forward(ctx, image, measured, options):
-        y, grad = _C.residual_step(image.contiguous(), measured.contiguous(), options)
+        # per-row sums of (A x - m)^2 come from the same call (FP64), so A x is never stored
+        rows, grad = _C.residual_loss_step(image.contiguous(), measured.contiguous(), options)
         ctx.save_for_backward(grad)
-        r = y - measured
-        return 0.5 * (r * r).sum()
+        return (0.5 * rows.sum()).to(image.dtype)
 
     @staticmethod
     def backward(ctx, gout):

@@ -303,9 +303,9 @@ hipError_t run_to_sino(const ctlib_plan* p, int dtype, const void* in, void* out
 
 // Bin tables for the image-output ops.
 hipError_t run_tables(const ctlib_plan* p, int in_dtype, const void* y, const void* m, const double* pre,
-                      double2* tabs, int B, hipStream_t s) {
+                      double2* tabs, int B, hipStream_t s, double* rowsq = nullptr) {
     KernelTimer t(K_AUX, 0, 0, s);
-    return launch_bin_tables(p->dev, in_dtype, y, m, pre, 1.0, tabs, B, s);
+    return launch_bin_tables(p->dev, in_dtype, y, m, pre, 1.0, tabs, B, s, rowsq);
 }
 
 // Image-output ops from prepared bin tables: projection_t and backprojection.
@@ -692,6 +692,11 @@ size_t ctlib_residual_workspace_bytes(const ctlib_plan* p, int64_t batch, int dt
 
 int ctlib_run_residual(const ctlib_plan* p, int dtype, const void* x, const void* m, void* y, void* grad,
                        int64_t batch, void* ws, size_t ws_bytes, void* stream) {
+    return ctlib_run_residual_loss(p, dtype, x, m, y, grad, nullptr, batch, ws, ws_bytes, stream);
+}
+
+int ctlib_run_residual_loss(const ctlib_plan* p, int dtype, const void* x, const void* m, void* y, void* grad,
+                            double* row_sq, int64_t batch, void* ws, size_t ws_bytes, void* stream) {
     int rc = check_run_args(p, CTLIB_OP_PRJ_T, dtype, x, grad, batch, ws, ws_bytes);
     if (rc) return rc;
     if (batch == 0) return CTLIB_OK;
@@ -719,7 +724,7 @@ int ctlib_run_residual(const ctlib_plan* p, int dtype, const void* x, const void
         // (a separate grid-wide sum measured faster than summing inside the per-row table pass)
         void* yv = y ? y : w + la.sp;
         e = run_to_sino(p, dt, x, yv, B, false, P.post_prj, la, w, s);
-        if (e == hipSuccess) e = run_tables(p, dt, yv, m, P.post_prj, tabs, B, s);
+        if (e == hipSuccess) e = run_tables(p, dt, yv, m, P.post_prj, tabs, B, s, row_sq);
         if (e == hipSuccess) e = run_to_image(p, dt, tabs, grad, B, false, 1.0, lb, w, s);
         if (e != hipSuccess) return hip_fail(e, "kernel launch");
         return CTLIB_OK;
@@ -731,7 +736,8 @@ int ctlib_run_residual(const ctlib_plan* p, int dtype, const void* x, const void
     }
     if (e == hipSuccess) {
         KernelTimer t(K_FWD, sweep_bytes(P, B, dt), 0, s);
-        e = launch_fwd_resid(P, dt, x, P.ny ? yl : nullptr, y, m, tabs, B, default_fwd_shape(P, false, p->fwd_G), s);
+        e = launch_fwd_resid(P, dt, x, P.ny ? yl : nullptr, y, m, tabs, B, default_fwd_shape(P, false, p->fwd_G), s,
+                             row_sq);
     }
     if (e == hipSuccess) e = run_to_image(p, dt, tabs, grad, B, false, 1.0, lb, w, s);
     if (e != hipSuccess) return hip_fail(e, "kernel launch");

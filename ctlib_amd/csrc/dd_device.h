@@ -208,13 +208,15 @@ __device__ __forceinline__ void lent_write(void* p, double c, double s) {
 
 // Bin values y_k = post_k (S_{k+1} - S_k) from the edge sums of a wave's VPW
 // views, formed with wave shifts; RESID also writes the residual bin tables of
-// projection_t, tabs[b][view] = cumulative {C, g} of g = (y - m) * post.
+// projection_t, tabs[b][view] = cumulative {C, g} of g = (y - m) * post, and, when
+// rowsq is not null, rowsq[b][view] = sum_k (y_k - m_k)^2 with y_k rounded to T.
 // EDGEX: D == 64*NEM, and the sum of the last edge D of view v sits in lane v of Sx.
 template <typename T, int VPW, int NEM, bool RESID, bool EDGEX = false>
 __device__ __forceinline__ void fwd_store(const DevPlan& P, const double (&S)[VPW][NEM], const int (&vw)[VPW], int grp,
                                           int b, int mode, int lane, int wave, T* __restrict__ y,
                                           const double* __restrict__ post, const T* __restrict__ m,
-                                          double2* __restrict__ tabs, double Sx = 0.0) {
+                                          double2* __restrict__ tabs, double Sx = 0.0,
+                                          double* __restrict__ rowsq = nullptr) {
     constexpr int G = 4 * VPW;
     const int nv = mode == MODE_Y ? P.ny : P.nx;
     const int D = P.D;
@@ -229,7 +231,7 @@ __device__ __forceinline__ void fwd_store(const DevPlan& P, const double (&S)[VP
         const double* pr = post ? post + (size_t)view * D : nullptr;
         const double* ev = P.edges + (size_t)view * (D + 1);
         double2* tb = RESID ? tabs + row * (D + 2) : nullptr;
-        double carry = 0.0;
+        double carry = 0.0, rsq = 0.0;
 #pragma unroll
         for (int i = 0; i < NEM; i++) {
             if (i >= ne) continue;  // wave-uniform
@@ -250,7 +252,10 @@ __device__ __forceinline__ void fwd_store(const DevPlan& P, const double (&S)[VP
                 double gk = 0.0, ek = 0.0, wg = 0.0;
                 if (k < D) {
                     if (y) y[row * D + o] = (T)yk;
-                    gk = (yk - (double)m[row * D + o]) * pr[k];
+                    const double mk = (double)m[row * D + o];
+                    gk = (yk - mk) * pr[k];
+                    const double rk = (double)(T)yk - mk;
+                    rsq = __builtin_fma(rk, rk, rsq);
                     ek = ev[k];
                     wg = (ev[k + 1] - ek) * gk;
                 }
@@ -262,6 +267,10 @@ __device__ __forceinline__ void fwd_store(const DevPlan& P, const double (&S)[VP
         if (RESID && lane == 0) {
             tb[0] = make_double2(0.0, 0.0);
             tb[D + 1] = make_double2(carry, 0.0);
+        }
+        if (RESID && rowsq) {
+            const double tot = readlane_d(wave_scan_incl(rsq, lane), 63);
+            if (lane == 0) rowsq[row] = tot;
         }
     }
 }
@@ -331,7 +340,8 @@ template <typename T, int VPW, int NEM, int M, bool WEIGHTED, bool RESID, bool F
 __device__ __forceinline__ void fwd_block(const DevPlan& P, const T* __restrict__ src, T* __restrict__ y, int b,
                                           int mode, int grp, const double* __restrict__ post,
                                           const T* __restrict__ m, double2* __restrict__ tabs,
-                                          unsigned char* __restrict__ smem, int lb = 0, int le = -1) {
+                                          unsigned char* __restrict__ smem, int lb = 0, int le = -1,
+                                          double* __restrict__ rowsq = nullptr) {
     constexpr int G = 4 * VPW;
     constexpr int NTAB = WEIGHTED ? G : 1;
     const int tid = threadIdx.x, lane = tid & 63;
@@ -454,7 +464,7 @@ __device__ __forceinline__ void fwd_block(const DevPlan& P, const T* __restrict_
         }
     }
 
-    fwd_store<T, VPW, NEM, RESID, EDGEX>(P, S, vw, grp, b, mode, lane, wave, y, post, m, tabs, Sx);
+    fwd_store<T, VPW, NEM, RESID, EDGEX>(P, S, vw, grp, b, mode, lane, wave, y, post, m, tabs, Sx, rowsq);
 }
 
 // Entries of a wave-private line table sit at j + j/8 (one 16-byte pad every 8):

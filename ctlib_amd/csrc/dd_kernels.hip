@@ -37,7 +37,7 @@ template <typename T, int VPW, int NEM, int M, bool WEIGHTED, bool RESID, bool F
 __global__ __launch_bounds__(256, (fwd_waves<T, NEM, M, WEIGHTED>())) void k_fwd(DevPlan P, const T* __restrict__ srcx, const T* __restrict__ srcy,
                                              T* __restrict__ y, int B, int ngx, int ngy,
                                              const double* __restrict__ post, const T* __restrict__ m,
-                                             double2* __restrict__ tabs, int nch) {
+                                             double2* __restrict__ tabs, int nch, double* __restrict__ rowsq) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int nwx = B * ngx;
     int w = xcd_remap(blockIdx.x, gridDim.x, (nwx + B * ngy) * nch);
@@ -51,7 +51,7 @@ __global__ __launch_bounds__(256, (fwd_waves<T, NEM, M, WEIGHTED>())) void k_fwd
     const int nl = mode == MODE_Y ? P.W : P.H;
     fwd_block<T, VPW, NEM, M, WEIGHTED, RESID, FULL, EDGEX>(P, mode == MODE_Y ? srcy : srcx,
                                                             y + (size_t)ch * B * P.V * P.D, b, mode, grp, post, m,
-                                                            tabs, smem, ch * nl / nch, (ch + 1) * nl / nch);
+                                                            tabs, smem, ch * nl / nch, (ch + 1) * nl / nch, rowsq);
 }
 
 // out[i] = sum_k parts[k * count + i], k < nch (partial sinograms of a line-chunked launch);
@@ -65,11 +65,13 @@ __global__ __launch_bounds__(256) void k_sum_parts(T* out, const T* parts, int n
     }
 }
 
-// Per-(b, view) bin tables {C_j, g_j} of g_k = ((y - m)[o(k)] * pre[k]) * scale, j = 0..D+1.
+// Per-(b, view) bin tables {C_j, g_j} of g_k = ((y - m)[o(k)] * pre[k]) * scale, j = 0..D+1;
+// with m and rowsq not null also rowsq[row] = sum_k (y - m)_k^2.
 template <typename TIn>
 __global__ __launch_bounds__(256) void k_bin_tables(DevPlan P, const TIn* __restrict__ y, const TIn* __restrict__ m,
                                                     const double* __restrict__ pre, double scale,
-                                                    double2* __restrict__ tabs, int nrows) {
+                                                    double2* __restrict__ tabs, int nrows,
+                                                    double* __restrict__ rowsq) {
     const int lane = threadIdx.x & 63;
     const int row = blockIdx.x * 4 + (threadIdx.x >> 6);  // b*V + v
     if (row >= nrows) return;                              // whole wave; no barriers below
@@ -82,7 +84,7 @@ __global__ __launch_bounds__(256) void k_bin_tables(DevPlan P, const TIn* __rest
     const double* pr = pre ? pre + (size_t)v * D : nullptr;
     double2* tb = tabs + (size_t)row * (D + 2);
     // 64 consecutive bins per step: coalesced loads and stores, one wave scan per step
-    double carry = 0.0;
+    double carry = 0.0, rsq = 0.0;
     for (int k0 = 0; k0 < D; k0 += 64) {
         const int k = k0 + lane;
         double g = 0.0, ek = 0.0, wg = 0.0;
@@ -90,6 +92,7 @@ __global__ __launch_bounds__(256) void k_bin_tables(DevPlan P, const TIn* __rest
             const int o = rev ? D - 1 - k : k;
             g = (double)yr[o];
             if (mr) g -= (double)mr[o];
+            rsq = __builtin_fma(g, g, rsq);
             if (pr) g *= pr[k];
             g *= scale;
             ek = e[k];
@@ -102,6 +105,10 @@ __global__ __launch_bounds__(256) void k_bin_tables(DevPlan P, const TIn* __rest
     if (lane == 0) {
         tb[0] = make_double2(0.0, 0.0);
         tb[D + 1] = make_double2(carry, 0.0);
+    }
+    if (rowsq && mr) {
+        const double tot = readlane_d(wave_scan_incl(rsq, lane), 63);
+        if (lane == 0) rowsq[row] = tot;
     }
 }
 
@@ -147,6 +154,7 @@ struct FwdArgs {
     const void* m;
     double2* tabs;
     int nch;
+    double* rowsq;
 };
 
 template <typename T, int VPW, int NEM, int M, bool WTD, bool RESID, bool FULL, bool EDGEX>
@@ -163,7 +171,7 @@ hipError_t fwd_launch_t(const DevPlan& P, const FwdArgs& a, hipStream_t s) {
         if (e != hipSuccess) return e;
     }
     hipLaunchKernelGGL(kern, dim3(grid_for(nwork)), dim3(NT), lds, s, P, (const T*)a.srcx, (const T*)a.srcy, (T*)a.y,
-                       a.B, ngx, ngy, a.post, (const T*)a.m, a.tabs, a.nch);
+                       a.B, ngx, ngy, a.post, (const T*)a.m, a.tabs, a.nch, a.rowsq);
     return hipGetLastError();
 }
 
@@ -221,16 +229,16 @@ hipError_t launch_fwd(const DevPlan& P, int dtype, const void* srcx, const void*
     const int nch = shape.nch < 1 ? 1 : shape.nch;
     if ((P.nx && nch > P.H) || (P.ny && nch > P.W)) return hipErrorInvalidValue;
     if (weighted && fwd_w_supported(P)) return launch_fwd_w(P, dtype, srcx, srcy, y, B, post, nch, s);
-    FwdArgs a{srcx, srcy, y, B, post, nullptr, nullptr, nch};
+    FwdArgs a{srcx, srcy, y, B, post, nullptr, nullptr, nch, nullptr};
     if (dtype == DT_F32) return fwd_dispatch<float>(P, a, weighted, false, shape.G, s);
     return fwd_dispatch<double>(P, a, weighted, false, shape.G, s);
 }
 
 hipError_t launch_fwd_resid(const DevPlan& P, int dtype, const void* srcx, const void* srcy, void* y, const void* m,
-                            double2* tabs, int B, FwdShape shape, hipStream_t s) {
+                            double2* tabs, int B, FwdShape shape, hipStream_t s, double* rowsq) {
     if (B <= 0 || P.nx + P.ny == 0) return hipSuccess;
     if ((P.nx && P.W > NT * 8) || (P.ny && P.H > NT * 8) || P.D > 2046) return hipErrorInvalidValue;
-    FwdArgs a{srcx, srcy, y, B, P.post_prj, m, tabs, 1};
+    FwdArgs a{srcx, srcy, y, B, P.post_prj, m, tabs, 1, rowsq};
     if (dtype == DT_F32) return fwd_dispatch<float>(P, a, false, true, shape.G, s);
     return fwd_dispatch<double>(P, a, false, true, shape.G, s);
 }
@@ -249,16 +257,16 @@ hipError_t launch_sum_parts(void* out, const void* parts, int dtype, int nch, si
 }
 
 hipError_t launch_bin_tables(const DevPlan& P, int in_dtype, const void* y, const void* m, const double* pre,
-                             double scale, double2* tabs, int B, hipStream_t s) {
+                             double scale, double2* tabs, int B, hipStream_t s, double* rowsq) {
     const int nrows = B * P.V;
     if (nrows <= 0) return hipSuccess;
     dim3 grid((nrows + 3) / 4);
     if (in_dtype == DT_F32)
         hipLaunchKernelGGL(k_bin_tables<float>, grid, dim3(256), 0, s, P, (const float*)y, (const float*)m, pre, scale,
-                           tabs, nrows);
+                           tabs, nrows, rowsq);
     else
         hipLaunchKernelGGL(k_bin_tables<double>, grid, dim3(256), 0, s, P, (const double*)y, (const double*)m, pre,
-                           scale, tabs, nrows);
+                           scale, tabs, nrows, rowsq);
     return hipGetLastError();
 }
 

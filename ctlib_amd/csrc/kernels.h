@@ -54,9 +54,10 @@ hipError_t launch_sum_parts(void* out, const void* parts, int dtype, int nch, si
 size_t fwd_lds_bytes(const DevPlan& P, bool weighted, int G);
 
 // projection fused with the residual bin tables of projection_t: writes y = A x (y may be null)
-// and tabs[b][v] = tables of g = (y - m) * post_prj, so that launch_adj yields A^T (A x - m).
+// and tabs[b][v] = tables of g = (y - m) * post_prj, so that launch_adj yields A^T (A x - m);
+// rowsq (may be null) receives sum_k (y - m)^2 per row [B][V], y rounded to the data type.
 hipError_t launch_fwd_resid(const DevPlan& P, int dtype, const void* srcx, const void* srcy, void* y, const void* m,
-                            double2* tabs, int B, FwdShape shape, hipStream_t s);
+                            double2* tabs, int B, FwdShape shape, hipStream_t s, double* rowsq = nullptr);
 
 // Weighted line -> bin operator (backprojection_t; post may be null) with wave-private line
 // tables, for lines of at most 1024 pixels (fwd_w_supported); launch_fwd dispatches to it.
@@ -64,9 +65,10 @@ bool fwd_w_supported(const DevPlan& P);
 hipError_t launch_fwd_w(const DevPlan& P, int dtype, const void* srcx, const void* srcy, void* y, int B,
                         const double* post, int nch, hipStream_t s);
 
-// Per (b, view) cumulative-integral tables {C_j, g_j}, j = 0..D+1, of g_k = (y - m)[o(k)] * pre[k] * scale.
+// Per (b, view) cumulative-integral tables {C_j, g_j}, j = 0..D+1, of g_k = (y - m)[o(k)] * pre[k] * scale;
+// with m and rowsq not null also rowsq[b][v] = sum_k (y - m)_k^2.
 hipError_t launch_bin_tables(const DevPlan& P, int in_dtype, const void* y, const void* m, const double* pre,
-                             double scale, double2* tabs, int B, hipStream_t s);
+                             double scale, double2* tabs, int B, hipStream_t s, double* rowsq = nullptr);
 
 // Bin -> pixel operator (projection_t; weighted = backprojection, pixel weight * wscale), both axes in
 // one launch.  X lines are written to outX [B][H][W]; Y lines to outYT [B][W][H] (bottom-to-top per column).

@@ -197,8 +197,10 @@ torch::Tensor fbp_t(torch::Tensor image, torch::Tensor options) {
     return run_op(image, options, CTLIB_OP_FBP_T, IMG_TO_SINO, "image");
 }
 
-// One least-squares step: returns (A x, A^T (A x - m)).
-std::vector<torch::Tensor> residual_step(torch::Tensor image, torch::Tensor measured, torch::Tensor options) {
+// One least-squares step: returns (A x, A^T (A x - m)); with_loss: (row sums of (A x - m)^2 as
+// [B, V] f64, A^T (A x - m)) instead, without materialising A x.
+std::vector<torch::Tensor> residual_common(torch::Tensor image, torch::Tensor measured, torch::Tensor options,
+                                           bool with_loss) {
     torch::Tensor x = check_input(image, "image");
     torch::Tensor m = check_input(measured, "measured");
     const int dt = dtype_code(x);
@@ -213,16 +215,26 @@ std::vector<torch::Tensor> residual_step(torch::Tensor image, torch::Tensor meas
         throw pybind11::value_error("image shape does not match the options");
     if (m.size(0) != B || m.size(2) != g.views || m.size(3) != g.dets)
         throw pybind11::value_error("measured sinogram shape does not match the options / batch");
-    torch::Tensor y = torch::empty({B, 1, g.views, g.dets}, x.options());
+    torch::Tensor first = with_loss ? torch::empty({B, g.views}, x.options().dtype(torch::kFloat64))
+                                    : torch::empty({B, 1, g.views, g.dets}, x.options());
     torch::Tensor grad = torch::empty({B, 1, g.height, g.width}, x.options());
-    if (B == 0) return {y, grad};
+    if (B == 0) return {first, grad};
     const size_t wsb = ctlib_residual_workspace_bytes(c.plan.get(), B, dt);
     torch::Tensor ws = torch::empty({(int64_t)wsb}, x.options().dtype(torch::kUInt8));
     hipStream_t s = c10::hip::getCurrentHIPStream((c10::DeviceIndex)device).stream();
-    int rc = ctlib_run_residual(c.plan.get(), dt, x.data_ptr(), m.data_ptr(), y.data_ptr(), grad.data_ptr(), B,
-                                ws.data_ptr(), wsb, (void*)s);
+    int rc = ctlib_run_residual_loss(c.plan.get(), dt, x.data_ptr(), m.data_ptr(),
+                                     with_loss ? nullptr : first.data_ptr(), grad.data_ptr(),
+                                     with_loss ? first.data_ptr<double>() : nullptr, B, ws.data_ptr(), wsb, (void*)s);
     if (rc) raise_status(rc, "residual_step");
-    return {y, grad};
+    return {first, grad};
+}
+
+std::vector<torch::Tensor> residual_step(torch::Tensor image, torch::Tensor measured, torch::Tensor options) {
+    return residual_common(image, measured, options, false);
+}
+
+std::vector<torch::Tensor> residual_loss_step(torch::Tensor image, torch::Tensor measured, torch::Tensor options) {
+    return residual_common(image, measured, options, true);
 }
 
 // Geometry of an options vector as the library parses it (for host-side code and tests).
@@ -386,6 +398,8 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m) {
     m.def("backprojection_sv", &backprojection_sv, "Per-view backprojection, [B,V,H,W] (HIP)",
           pybind11::arg("projection"), pybind11::arg("options"), pybind11::arg("any_scan_type") = false);
     m.def("residual_step", &residual_step, "(A x, A^T (A x - m)) in one call (HIP)");
+    m.def("residual_loss_step", &residual_loss_step,
+          "(row sums of (A x - m)^2 [B, V] f64, A^T (A x - m)) in one call (HIP)");
     m.def("laplacian", &laplacian, "kNN graph weights as a sparse (N, N) tensor (HIP)");
     m.def("parse_options", &parse_options, "Parse an options vector");
     m.def("workspace_bytes", &workspace_bytes, "Scratch bytes of one op");

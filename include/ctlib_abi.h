@@ -94,6 +94,14 @@ int ctlib_run(const ctlib_plan* plan, int op, int dtype, const void* in, void* o
 int ctlib_run_residual(const ctlib_plan* plan, int dtype, const void* x, const void* m, void* y,
                        void* grad, int64_t batch, void* ws, size_t ws_bytes, void* stream);
 size_t ctlib_residual_workspace_bytes(const ctlib_plan* plan, int64_t batch, int dtype);
+/* As ctlib_run_residual, and row_sq[b * views + v] = sum over the bins of row (b, v) of
+ * (y - m)^2 in FP64, y rounded to the data type first, so that 0.5 * sum(row_sq) is the
+ * least-squares loss without another pass over y and m.  row_sq: device array of
+ * batch * views doubles, or NULL.  Same workspace as ctlib_run_residual.  No reference
+ * counterpart. */
+int ctlib_run_residual_loss(const ctlib_plan* plan, int dtype, const void* x, const void* m, void* y,
+                            void* grad, double* row_sq, int64_t batch, void* ws, size_t ws_bytes,
+                            void* stream);
 
 /* Launch-shape override chosen by the autotuner: views per block of the
  * line->bin kernel (4 or 8; 0 = automatic; 8 applies when D < 768).  Set it before the plan is

@@ -159,3 +159,41 @@ def test_checkpointing_memory(dev):
     assert peaks[True] < 0.6 * peaks[False], peaks
     torch.testing.assert_close(grads[True][0], grads[False][0], rtol=1e-4, atol=0)
     torch.testing.assert_close(grads[True][1], grads[False][1], rtol=1e-4, atol=1e-6)
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+@pytest.mark.parametrize("chunks", [0, 1, 3])
+@pytest.mark.parametrize("st", [0, 1, 2])
+def test_least_squares_loss_and_grad(dev, st, chunks, dtype):
+    """least_squares: the loss 0.5 * ||A x - m||^2 (from the library's per-row FP64 sums) and its
+    gradient A^T (A x - m) against the FP64 oracle, for the forward fused with the residual tables
+    (1 line chunk) and the split forward (automatic and 3 chunks)."""
+    from ctlib_amd import _C
+    o = small(st)
+    g = O.parse(o)
+    rng = np.random.default_rng(70 + st)
+    x = rng.random((2, 1, g["height"], g["width"]))
+    m = rng.random((2, 1, g["views"], g["dets"]))
+    ot = torch.from_numpy(o)
+    print(f"[case] least squares st={st} chunks={chunks} {dtype}", flush=True)
+    xt = torch.from_numpy(x).to(dev, dtype).requires_grad_(True)
+    mt = torch.from_numpy(m).to(dev, dtype)
+    try:
+        _C.set_plan_chunks(ot, dev.index or 0, chunks, 0)
+        loss = autograd.least_squares(xt, mt, ot)
+        loss.backward()
+        y, _ = ctlib_amd.residual_step(xt.detach(), mt, ot)
+    finally:
+        _C.set_plan_chunks(ot, dev.index or 0, 0, 0)
+    xr = xt.detach().cpu().double().numpy()[:, 0]
+    mr = mt.cpu().double().numpy()[:, 0]
+    r = O.apply("projection", o.astype(np.float64), xr) - mr
+    tol = 1e-6 if dtype == torch.float32 else 1e-12
+    assert loss.dtype == dtype
+    assert abs(loss.item() - 0.5 * float((r * r).sum())) <= tol * 0.5 * float((r * r).sum())
+    # the same sum from the stored A x (rounded to the data type) agrees to that rounding
+    r_dev = (y.double() - mt.double()).cpu().numpy()
+    assert abs(loss.item() - 0.5 * float((r_dev * r_dev).sum())) <= tol * 0.5 * float((r * r).sum())
+    g_ref = O.apply("projection_t", o.astype(np.float64), r)
+    got = xt.grad.cpu().double().numpy()[:, 0]
+    assert np.linalg.norm(got - g_ref) / np.linalg.norm(g_ref) <= (1e-5 if dtype == torch.float32 else 1e-7)
