@@ -411,6 +411,7 @@ int ctlib_plan_create(const ctlib_geom* geom, int device, ctlib_plan** out) {
     const size_t o_yv = take(sizeof(int) * (H.yviews.size() + 1));
     const size_t o_taps = take(sizeof(double) * H.ramp_taps.size());
     const size_t o_pw = take(sizeof(double) * H.ramp_preweight.size());
+    const size_t o_lkd = take(sizeof(double) * H.lookup_d.size());
     const size_t total = off;
 
     DeviceGuard guard(device);
@@ -435,6 +436,7 @@ int ctlib_plan_create(const ctlib_geom* geom, int device, ctlib_plan** out) {
         {o_yv, H.yviews.data(), sizeof(int) * H.yviews.size()},
         {o_taps, H.ramp_taps.data(), sizeof(double) * H.ramp_taps.size()},
         {o_pw, H.ramp_preweight.data(), sizeof(double) * H.ramp_preweight.size()},
+        {o_lkd, H.lookup_d.data(), sizeof(double) * H.lookup_d.size()},
     };
     for (const Copy& c : copies) {
         if (!c.bytes) continue;
@@ -455,6 +457,7 @@ int ctlib_plan_create(const ctlib_geom* geom, int device, ctlib_plan** out) {
     P.yviews = reinterpret_cast<const int*>(base + o_yv);
     P.taps = reinterpret_cast<const double*>(base + o_taps);
     P.preweight = reinterpret_cast<const double*>(base + o_pw);
+    P.lookup_d = reinterpret_cast<const double*>(base + o_lkd);
     P.nx = (int)H.xviews.size();
     P.ny = (int)H.yviews.size();
     P.V = V;

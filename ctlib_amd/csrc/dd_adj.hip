@@ -77,14 +77,14 @@ __global__ __launch_bounds__(256) void k_bprj_sv(DevPlan P, const double2* __res
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const ViewParams& vp = P.vp[v];
-    const Lookup lu = view_lookup(P, vp);
+    const Lookup lu = view_lookup(P, v);
     const float hi = (float)(D + 1);
     const LineCoef* lc = P.lines + (size_t)v * P.max_lines;
     T* img = out + (size_t)item * H * W;
     auto wgt = [&](const LineCoef& c, int p) {
         const double u = __builtin_fma(c.w1, (double)p, c.w0);
         const double d = __builtin_fma(u, u, c.w2);
-        return c.inv_s * wscale * (double)__builtin_amdgcn_rcpf((float)d);
+        return c.inv_s * wscale * (sizeof(T) == 8 ? rcp_refined(d) : (double)__builtin_amdgcn_rcpf((float)d));
     };
     if (vp.xmode) {
         for (int r = wave; r < H; r += NT / 64) {
@@ -92,7 +92,7 @@ __global__ __launch_bounds__(256) void k_bprj_sv(DevPlan P, const double2* __res
             T* dst = img + (size_t)r * W;
             double g[NPE];
 #pragma unroll
-            for (int i = 0; i < NPE; i++) g[i] = eval_G<GEO>(t, c.a, c.s, lane + 64 * i, lu, hi);
+            for (int i = 0; i < NPE; i++) g[i] = eval_G<GEO, sizeof(T) == 8>(t, c.a, c.s, lane + 64 * i, lu, hi);
 #pragma unroll
             for (int i = 0; i < NPE; i++) {
                 double right = shfl_next_d(g[i]);
@@ -110,13 +110,13 @@ __global__ __launch_bounds__(256) void k_bprj_sv(DevPlan P, const double2* __res
             const int col = c0 + lane;
             const bool ok = col < W;
             const LineCoef c = lc[ok ? col : W - 1];
-            double gprev = eval_G<GEO>(t, c.a, c.s, 0, lu, hi);
+            double gprev = eval_G<GEO, sizeof(T) == 8>(t, c.a, c.s, 0, lu, hi);
             T* dst = img + (size_t)(H - 1) * W + col;
             int i = 0;
             for (; i + SV_UNROLL <= H; i += SV_UNROLL) {
                 double g[SV_UNROLL];
 #pragma unroll
-                for (int u = 0; u < SV_UNROLL; u++) g[u] = eval_G<GEO>(t, c.a, c.s, i + u + 1, lu, hi);
+                for (int u = 0; u < SV_UNROLL; u++) g[u] = eval_G<GEO, sizeof(T) == 8>(t, c.a, c.s, i + u + 1, lu, hi);
 #pragma unroll
                 for (int u = 0; u < SV_UNROLL; u++) {
                     const double left = u ? g[u - 1] : gprev;
@@ -125,7 +125,7 @@ __global__ __launch_bounds__(256) void k_bprj_sv(DevPlan P, const double2* __res
                 gprev = g[SV_UNROLL - 1];
             }
             for (; i < H; i++) {
-                const double gi = eval_G<GEO>(t, c.a, c.s, i + 1, lu, hi);
+                const double gi = eval_G<GEO, sizeof(T) == 8>(t, c.a, c.s, i + 1, lu, hi);
                 if (ok) dst[-(ptrdiff_t)i * W] = (T)(wgt(c, i) * (gi - gprev));
                 gprev = gi;
             }

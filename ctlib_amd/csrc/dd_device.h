@@ -52,6 +52,14 @@ __device__ __forceinline__ int xcd_remap(int bid, int nblocks, int nwork) {
 
 inline int grid_for(int nwork) { return ((nwork + 7) / 8) * 8; }
 
+// 1/d for fp64 data: the fp32 hardware reciprocal (relative error ~1e-7) refined by one
+// Newton step, r + r (1 - d r), to ~1e-14.  Used for squared source-pixel distances and
+// lookup denominators, which valid geometries keep well inside the fp32 normal range.
+__device__ __forceinline__ double rcp_refined(double d) {
+    const double r = (double)__builtin_amdgcn_rcpf((float)d);
+    return __builtin_fma(r, __builtin_fma(-d, r, 1.0), r);
+}
+
 // Index of a cumulative table from a float coordinate, clamped to [0, hi].
 __device__ __forceinline__ int table_index(float q, float hi) { return (int)__builtin_amdgcn_fmed3f(q, 0.0f, hi); }
 
@@ -422,7 +430,8 @@ __device__ __forceinline__ void fwd_block(const DevPlan& P, const T* __restrict_
                         const double u = __builtin_fma(w1, (double)(p0 + i), w0);
                         const double d = __builtin_fma(u, u, w2);
                         // rounded to the slope's storage precision before the prefix sum
-                        val[t][i] = (double)(T)((double)xv[i] * (is * (double)__builtin_amdgcn_rcpf((float)d)));
+                        const double rd = sizeof(T) == 8 ? rcp_refined(d) : (double)__builtin_amdgcn_rcpf((float)d);
+                        val[t][i] = (double)(T)((double)xv[i] * (is * rd));
                     }
                 }
             } else {
@@ -486,7 +495,7 @@ __host__ __device__ inline size_t fwd_w_block_lds(int nmax) { return (size_t)4 *
 // 64*PPL pixels, so no lane needs a bounds predicate.  The pixel weight
 // 1/(s ((w0 + w1 p)^2 + w2)) is formed in the data precision (fp32 data: fp32
 // arithmetic with a fp32 reciprocal, about 1e-7 relative; fp64 data: fp64
-// arithmetic with a fp32 reciprocal), and the weighted value is rounded to the
+// arithmetic with a Newton-refined reciprocal), and the weighted value is rounded to the
 // data type before the fp64 prefix sum, as in fwd_block with WEIGHTED.
 // Lines [lb, le) of the axis contribute (le < 0: all), as in fwd_block.
 template <typename T, int NEM, int PPL, bool FULL>
@@ -552,7 +561,7 @@ __device__ __forceinline__ void fwd_block_w(const DevPlan& P, const T* __restric
             for (int k = 0; k < PPL; k++) {
                 const double u = __builtin_fma(w1, (double)(p0 + k), w0);
                 const double d = __builtin_fma(u, u, w2);
-                const double v = (double)xv[k] * (is * (double)__builtin_amdgcn_rcpf((float)d));
+                const double v = (double)xv[k] * (is * rcp_refined(d));
                 val[k] = (FULL || p0 + k < n) ? v : 0.0;
             }
         }
@@ -591,9 +600,11 @@ struct Lookup {
     float lk[4];      // Moebius (ED, EA) or affine (parallel) coefficients
     float ea_sc, ea_off;
     float c[6], T;    // fan-EA odd polynomial for atan on |t| <= T (GEO_EAP)
+    double lkd[6];    // lk0..lk3, ea_sc, ea_off in fp64 (fp64 data)
 };
 
-__device__ __forceinline__ Lookup view_lookup(const DevPlan& P, const ViewParams& vp) {
+__device__ __forceinline__ Lookup view_lookup(const DevPlan& P, int v) {
+    const ViewParams& vp = P.vp[v];
     Lookup L;
 #pragma unroll
     for (int k = 0; k < 4; k++) L.lk[k] = vp.lk[k];
@@ -602,16 +613,35 @@ __device__ __forceinline__ Lookup view_lookup(const DevPlan& P, const ViewParams
 #pragma unroll
     for (int k = 0; k < 6; k++) L.c[k] = P.ea_c[k];
     L.T = P.ea_T;
+    const double* d = P.lookup_d + (size_t)v * 6;
+#pragma unroll
+    for (int k = 0; k < 6; k++) L.lkd[k] = d[k];
     return L;
 }
 
 // G(pi_p) of pixel edge p of a line with edges pi_p = a + s*p.  GEO_EA takes the
 // arctangent with atanf; GEO_EAP with the plan's odd polynomial (t clamped to
-// [-T, T], beyond which the index clamps to an end entry anyway).
-template <int GEO>
+// [-T, T], beyond which the index clamps to an end entry anyway).  F64Q (fp64
+// data): the table index comes from the fp64 coefficients in fp64 arithmetic
+// (fan-EA with atan), since an fp32 index near a bin boundary picks the
+// neighbouring segment, an error of ~1e-4 bin widths times the jump in slope.
+template <int GEO, bool F64Q = false>
 __device__ __forceinline__ double eval_G(const double2* __restrict__ t, double a, double s, int p, const Lookup& L,
                                          float hi) {
     const double pi = __builtin_fma(s, (double)p, a);
+    if (F64Q) {
+        double qd;
+        if (GEO == GEO_PARA) {
+            qd = __builtin_fma(L.lkd[0], pi, L.lkd[1]);
+        } else {
+            const double num = __builtin_fma(L.lkd[0], pi, L.lkd[1]);
+            const double den = __builtin_fma(L.lkd[2], pi, L.lkd[3]);
+            qd = num * rcp_refined(den);
+            if (GEO == GEO_EA || GEO == GEO_EAP) qd = __builtin_fma(atan(qd), L.lkd[4], L.lkd[5]);
+        }
+        const double2 c = t[table_index_d(qd, (int)hi)];
+        return __builtin_fma(pi, c.y, c.x);
+    }
     const float pf = (float)pi;
     float q;
     if (GEO == GEO_PARA) {
@@ -716,11 +746,10 @@ __device__ __forceinline__ void adj_block(const DevPlan& P, const double2* __res
         }
         const double2* t = stab + cur * tabn;
         const double* cf = scf + cur * L * LC_WORDS;
-        const ViewParams& vp = P.vp[v];
-        const Lookup lu = view_lookup(P, vp);
+        const Lookup lu = view_lookup(P, v);
         double gx = 0.0;  // EXACT: G at edge n of line lx (lanes >= LPW: unused)
         if (EXACT) {
-            gx = eval_G<GEO>(t, cf[lx * LC_WORDS + LC_A0], cf[lx * LC_WORDS + LC_S], n, lu, hi);
+            gx = eval_G<GEO, sizeof(T) == 8>(t, cf[lx * LC_WORDS + LC_A0], cf[lx * LC_WORDS + LC_S], n, lu, hi);
             if (!WEIGHTED) accx += gx;
         }
 #pragma unroll
@@ -734,14 +763,14 @@ __device__ __forceinline__ void adj_block(const DevPlan& P, const double2* __res
             // so their LDS reads overlap.
             if (!WEIGHTED) {
 #pragma unroll
-                for (int i = 0; i < NG; i++) acc[q][i] += eval_G<GEO>(t, a, s, lane + 64 * i, lu, hi);
+                for (int i = 0; i < NG; i++) acc[q][i] += eval_G<GEO, sizeof(T) == 8>(t, a, s, lane + 64 * i, lu, hi);
             } else {
                 const double w0 = cf[lq * LC_WORDS + LC_W0], w1 = cf[lq * LC_WORDS + LC_W1];
                 const double w2 = cf[lq * LC_WORDS + LC_W2];
                 const double ws = cf[lq * LC_WORDS + LC_INVS] * wscale;
                 // Pixel weight ws / ((w0 + w1 p)^2 + w2).  fp32 storage: evaluated in fp32
                 // (relative error ~2e-7, below the output's rounding); fp64 storage: u and
-                // d in fp64, fp32 reciprocal.
+                // d in fp64, Newton-refined reciprocal.
                 const float w0f = (float)w0, w1f = (float)w1, w2f = (float)w2, wsf = (float)ws;
                 auto wgt = [&](int p) {
                     if (sizeof(T) == 4) {
@@ -750,14 +779,14 @@ __device__ __forceinline__ void adj_block(const DevPlan& P, const double2* __res
                     }
                     const double u = __builtin_fma(w1, (double)p, w0);
                     const double d = __builtin_fma(u, u, w2);
-                    return ws * (double)__builtin_amdgcn_rcpf((float)d);
+                    return ws * rcp_refined(d);
                 };
                 // right edge of pixel lane + 64 (i-1): edge lane+1 of group i-1, or for
                 // lane 63 edge 0 of group i; both come from the rotations of the groups
                 double gprev = 0.0, rprev = 0.0;
 #pragma unroll
                 for (int i = 0; i < NG; i++) {
-                    const double gi = eval_G<GEO>(t, a, s, lane + 64 * i, lu, hi);
+                    const double gi = eval_G<GEO, sizeof(T) == 8>(t, a, s, lane + 64 * i, lu, hi);
                     const double ri = rot_next_d(gi);
                     if (i > 0) {
                         const double right = lane == 63 ? ri : rprev;

@@ -20,6 +20,7 @@ struct DevPlan {
     const int* yviews;        // [ny]
     const double* taps;       // [2D-1]
     const double* preweight;  // [D]
+    const double* lookup_d;   // [V][6] lookup coefficients in fp64 (used for fp64 data)
     int nx, ny;
     int V, D, W, H, max_lines;
     int scan_type;

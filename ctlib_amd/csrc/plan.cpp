@@ -140,6 +140,7 @@ int build_plan(const Geom& g, HostPlan* P, std::string* why) {
     P->post_prj.assign((size_t)V * D, 0.0);
     P->post_bprj_t.assign((size_t)V * D, 1.0);
     P->lines.assign((size_t)V * P->max_lines, LineCoef{});
+    P->lookup_d.assign((size_t)V * 6, 0.0);
     P->xviews.clear();
     P->yviews.clear();
 
@@ -209,33 +210,38 @@ int build_plan(const Geom& g, HostPlan* P, std::string* why) {
             }
         }
 
-        // Axis coordinate -> (internal bin index + 1).
+        // Axis coordinate -> (internal bin index + 1): fp64 coefficients for fp64 data,
+        // rounded to fp32 in the view record for fp32 data.
+        double* lkd = &P->lookup_d[(size_t)v * 6];  // lk0..lk3, ea_sc, ea_off
         if (g.scan_type == SCAN_FAN_ED) {
             const double K = hD + 1 - sig * vs / vd;
             if (xmode) {
-                vp.lk[0] = (float)(sig * R * cs / vd + K * sn);
-                vp.lk[1] = (float)(K * R);
-                vp.lk[2] = (float)sn;
-                vp.lk[3] = (float)R;
+                lkd[0] = sig * R * cs / vd + K * sn;
+                lkd[1] = K * R;
+                lkd[2] = sn;
+                lkd[3] = R;
             } else {
-                vp.lk[0] = (float)(sig * R * sn / vd - K * cs);
-                vp.lk[1] = (float)(K * R);
-                vp.lk[2] = (float)(-cs);
-                vp.lk[3] = (float)R;
+                lkd[0] = sig * R * sn / vd - K * cs;
+                lkd[1] = K * R;
+                lkd[2] = -cs;
+                lkd[3] = R;
             }
         } else if (g.scan_type == SCAN_FAN_EA) {
-            vp.lk[0] = (float)(xmode ? cs : sn);
-            vp.lk[1] = 0.0f;
-            vp.lk[2] = (float)(xmode ? sn : -cs);
-            vp.lk[3] = (float)R;
-            vp.ea_sc = (float)(sig / g.dDet);
-            vp.ea_off = (float)(hD + 1 - sig * g.binshift / g.dDet);
+            lkd[0] = xmode ? cs : sn;
+            lkd[1] = 0.0;
+            lkd[2] = xmode ? sn : -cs;
+            lkd[3] = R;
+            lkd[4] = sig / g.dDet;
+            lkd[5] = hD + 1 - sig * g.binshift / g.dDet;
         } else {
-            vp.lk[0] = (float)(sig * (xmode ? cs : sn) / g.dDet);
-            vp.lk[1] = (float)(hD + 1 - sig * g.binshift / g.dDet);
-            vp.lk[2] = 0.0f;
-            vp.lk[3] = 1.0f;
+            lkd[0] = sig * (xmode ? cs : sn) / g.dDet;
+            lkd[1] = hD + 1 - sig * g.binshift / g.dDet;
+            lkd[2] = 0.0;
+            lkd[3] = 1.0;
         }
+        for (int k = 0; k < 4; k++) vp.lk[k] = (float)lkd[k];
+        vp.ea_sc = (float)lkd[4];
+        vp.ea_off = (float)lkd[5];
 
         // Per-line affine pixel-edge maps and pixel weights.
         LineCoef* lc = &P->lines[(size_t)v * P->max_lines];

@@ -54,6 +54,7 @@ struct HostPlan {
     std::vector<double> post_prj;        // [V][D]   projection factor dImg/(c_k w_k) (fan) or dImg/dDet (para)
     std::vector<double> post_bprj_t;     // [V][D]   backprojection_t factor: 1, or dImg/c_k for fan-EA
     std::vector<LineCoef> lines;         // [V][max_lines]
+    std::vector<double> lookup_d;        // [V][6] ViewParams lk0..lk3, ea_sc, ea_off in fp64
     std::vector<int> xviews, yviews;     // view indices per sweep axis, ascending
     std::vector<double> ramp_taps;       // [2D-1] RL taps h(-(D-1)..D-1)
     std::vector<double> ramp_preweight;  // [D] fbp pre-weight (output bin order)
