@@ -362,20 +362,24 @@ def test_line_and_view_chunks(dev, case, nch, dt):
         assert rel_l2(got["residual"][:, 0], g_ref) <= tol, ("residual", B)
 
 
-@pytest.mark.parametrize("cfg", [0, 2])
+@pytest.mark.parametrize("dt", ["f32", "f64"])
+@pytest.mark.parametrize("cfg", [0, 1, 2])
 @pytest.mark.parametrize("op", ["projection_t", "backprojection", "fbp"])
-def test_full_geometry_f64_random(dev, cfg, op):
-    """fp64 data on the full parallel 256^2 and fan-EA 512^2 geometries with uniform random
-    sinograms (fbp's ramp makes the output cancel heavily): the bin->pixel kernels take their
-    table index from fp64 lookup coefficients, so the result holds the fp64 tolerance."""
+def test_full_geometry_random(dev, cfg, op, dt):
+    """Uniform random sinograms on the full parallel 256^2, fan-ED 512^2 and fan-EA 512^2
+    geometries (fbp's ramp makes the output cancel heavily).  fp64 data: the bin->pixel kernels
+    take their table index from fp64 lookup coefficients, so the result holds the fp64
+    tolerance; fp32 data holds the fp32 one with the fp32 index."""
     o = phantom.CONFIGS[cfg]["options"].astype(np.float64)
     g = O.parse(o)
     rng = np.random.default_rng(300 + cfg)
     y = rng.random((1, 1, g["views"], g["dets"]))
-    print(f"[case] f64 random config{cfg} op={op}", flush=True)
+    if dt == "f32":
+        y = y.astype(np.float32)
+    print(f"[case] {dt} random config{cfg} op={op}", flush=True)
     got = gpu_op(op, y, o, dev)
-    ref = oracle_op(op, y, o)
-    assert got.dtype == np.float64
+    ref = oracle_op(op, y.astype(np.float64), o)
+    assert got.dtype == y.dtype
     err = rel_l2(got, ref)
     print(f"  rel_l2={err:.3e}")
-    assert err <= F64_TOL, err
+    assert err <= (F64_TOL if dt == "f64" else F32_TOL), err
