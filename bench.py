@@ -210,6 +210,52 @@ def workload_ops(args):
     return out
 
 
+def workload_small(args):
+    """Small batches: every DD operator at batch 1 (and 4 on fan-ED 512^2) on the parallel 256^2,
+    fan-ED 512^2 and fan-ED 1024^2 geometries, with the launches split automatically (line chunks
+    for line->bin, view chunks for bin->pixel) and unsplit (1 chunk), fp32."""
+    import torch
+
+    import ctlib_amd
+    from ctlib_amd import _C, phantom
+
+    dev = torch.device("cuda:0")
+    names = ("projection", "projection_t", "backprojection", "backprojection_t", "residual_step")
+    img_in = ("projection", "backprojection_t")
+    out = {"metric": "small-batch ms per call, split (automatic chunks) vs unsplit", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "dtype": "f32", "data": "synthetic", "configs": {}}
+    for cfg, B in ((0, 1), (1, 1), (1, 4), (4, 1)):
+        o = phantom.CONFIGS[cfg]["options"]
+        ot = torch.from_numpy(o)
+        g = ctlib_amd.parse_options(ot)
+        x = torch.from_numpy(phantom.phantom_batch(B, g["height"], g["width"], config=cfg, scale=0.02)).to(dev)
+        y = ctlib_amd.projection(x, ot)
+        rows = {}
+        for name in names:
+            row = {}
+            for label, chunks in (("split", 0), ("unsplit", 1)):
+                _C.set_plan_chunks(ot, dev.index or 0, chunks, chunks)
+                if name == "residual_step":
+                    fn = lambda: ctlib_amd.residual_step(x, y, ot)  # noqa: E731
+                else:
+                    f, inp = getattr(ctlib_amd, name), (x if name in img_in else y)
+                    fn = lambda: f(inp, ot)  # noqa: E731
+                for _ in range(args.warmup):
+                    fn()
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                for _ in range(args.steps):
+                    fn()
+                torch.cuda.synchronize()
+                row[label + "_ms"] = round((time.perf_counter() - t0) * 1e3 / args.steps, 4)
+            _C.set_plan_chunks(ot, dev.index or 0, 0, 0)
+            row["speedup"] = round(row["unsplit_ms"] / row["split_ms"], 2)
+            rows[name] = row
+        kind = {0: "parallel", 1: "fan-ED", 4: "fan-ED"}[cfg]
+        out["configs"][f"{kind} {g['height']}^2 {g['views']}x{g['dets']} B={B}"] = rows
+    return out
+
+
 def workload_laplacian(args):
     """laplacian (kNN graph weights) on N=16384 points of dimension 49 (7x7 patches), k=16, fp32.
     Compute-bound: 2*N^2*d flops of distances (VALU FMA chains, no N x N matrix in memory)."""
@@ -377,10 +423,12 @@ def main():
     ap.add_argument("--paths", default="plain,autograd,graph")
     ap.add_argument("--calib-copy", action="store_true",
                     help="run one 256 MiB device copy first (reference kernel for HBM counter scaling)")
-    ap.add_argument("--workload", default="metric", choices=("metric", "hetero", "unroll", "sv", "ops", "laplacian"),
+    ap.add_argument("--workload", default="metric", choices=("metric", "hetero", "unroll", "sv", "ops", "small",
+                                                              "laplacian"),
                     help="metric: the headline fan-ED B=32 step (default); hetero: 64-slice batch; "
                          "unroll: 50-iteration checkpointed reconstruction; sv: backprojection_sv; "
-                         "ops: every public operator; laplacian: kNN graph weights")
+                         "ops: every public operator; small: batch 1 / 4, split vs unsplit launches; "
+                         "laplacian: kNN graph weights")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only: the projector would shard over slices "
@@ -400,6 +448,9 @@ def main():
         return
     if args.workload == "ops":
         print(json.dumps(workload_ops(args)), flush=True)
+        return
+    if args.workload == "small":
+        print(json.dumps(workload_small(args)), flush=True)
         return
     if args.workload == "laplacian":
         print(json.dumps(workload_laplacian(args)), flush=True)

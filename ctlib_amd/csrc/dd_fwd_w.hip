@@ -100,6 +100,7 @@ hipError_t by_nem(const DevPlan& P, const Args& a, hipStream_t s) {
     if (ne <= 16) return by_ppl<T, 16>(P, a, s);
     if (ne <= 20) return by_ppl<T, 20>(P, a, s);
     if (ne <= 24) return by_ppl<T, 24>(P, a, s);
+    if (ne <= 26) return by_ppl<T, 26>(P, a, s);  // 1536 bins: 25 slots
     if (ne <= 32) return by_ppl<T, 32>(P, a, s);
     return hipErrorInvalidValue;
 }
