@@ -94,6 +94,7 @@ template <typename T>
 hipError_t by_nem(const DevPlan& P, const Args& a, hipStream_t s) {
     const int ne = (P.D + 1 + 63) / 64;
     if (ne <= 4) return by_ppl<T, 4>(P, a, s);
+    if (ne <= 6) return by_ppl<T, 6>(P, a, s);
     if (ne <= 8) return by_ppl<T, 8>(P, a, s);
     if (ne <= 12) return by_ppl<T, 12>(P, a, s);
     if (ne <= 14) return by_ppl<T, 14>(P, a, s);

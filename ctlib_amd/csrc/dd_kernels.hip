@@ -183,6 +183,7 @@ hipError_t fwd_m(const DevPlan& P, const FwdArgs& a, hipStream_t s) {
     const int m = (n + NT - 1) / NT;
     // every swept line (rows of X views, columns of Y views) exactly NT * M pixels
     auto full = [&](int M) { return (P.nx == 0 || P.W == NT * M) && (P.ny == 0 || P.H == NT * M); };
+    if (m <= 1 && full(1)) return fwd_launch_t<T, VPW, NEM, 1, WTD, RESID, true, EDGEX>(P, a, s);
     if (m <= 2) return full(2) ? fwd_launch_t<T, VPW, NEM, 2, WTD, RESID, true, EDGEX>(P, a, s)
                                : fwd_launch_t<T, VPW, NEM, 2, WTD, RESID, false, EDGEX>(P, a, s);
     if (m <= 4) return full(4) ? fwd_launch_t<T, VPW, NEM, 4, WTD, RESID, true, EDGEX>(P, a, s)
@@ -200,6 +201,7 @@ hipError_t fwd_ne(const DevPlan& P, const FwdArgs& a, hipStream_t s) {
     if (!WTD && VPW == 1 && P.D == 1024) return fwd_m<T, 1, 16, WTD, RESID, true>(P, a, s);
     if (!WTD && VPW == 1 && P.D == 1536) return fwd_m<T, 1, 24, WTD, RESID, true>(P, a, s);
     if (ne <= 4) return fwd_m<T, VPW, 4, WTD, RESID>(P, a, s);
+    if (ne <= 6) return fwd_m<T, VPW, 6, WTD, RESID>(P, a, s);
     if (ne <= 8) return fwd_m<T, VPW, 8, WTD, RESID>(P, a, s);
     if (ne <= 12) return fwd_m<T, VPW, 12, WTD, RESID>(P, a, s);
     if (VPW != 1) return hipErrorInvalidValue;
