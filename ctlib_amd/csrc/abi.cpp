@@ -13,6 +13,7 @@
 #include <hip/hip_runtime.h>
 #include <rocprofiler-sdk-roctx/roctx.h>
 
+#include <atomic>
 #include <cstring>
 #include <mutex>
 #include <new>
@@ -30,9 +31,11 @@ struct ctlib_plan {
     int device = 0;
     void* dmem = nullptr;
     DevPlan dev{};
-    int fwd_G = 0;  // autotuned views per line->bin block (0 = automatic)
-    int adj_nch = 0;  // view chunks of the bin->pixel kernel (0 = automatic)
-    int fwd_nch = 0;  // line chunks of the line->bin kernel (0 = automatic)
+    // Launch-shape overrides; each call reads them once (Tuning) so that the workspace it
+    // checks and the launches it makes agree even if another thread changes them.
+    std::atomic<int> fwd_G{0};    // autotuned views per line->bin block (0 = automatic)
+    std::atomic<int> adj_nch{0};  // view chunks of the bin->pixel kernel (0 = automatic)
+    std::atomic<int> fwd_nch{0};  // line chunks of the line->bin kernel (0 = automatic)
 };
 
 namespace ctlib {
@@ -81,6 +84,16 @@ struct DeviceGuard {
     }
 };
 
+// The plan's launch-shape overrides as one call sees them.
+struct Tuning {
+    int fwd_G, adj_nch, fwd_nch;
+};
+
+Tuning snap(const ctlib_plan* p) {
+    return Tuning{p->fwd_G.load(std::memory_order_relaxed), p->adj_nch.load(std::memory_order_relaxed),
+                  p->fwd_nch.load(std::memory_order_relaxed)};
+}
+
 inline size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
 
 size_t dsize(int dtype) { return dtype == CTLIB_F64 ? sizeof(double) : sizeof(float); }
@@ -106,13 +119,13 @@ void to_geom(const ctlib_geom* in, Geom* g) {
 // 512^2 images: batch 1 runs 5x faster with 12 chunks, batch 32 5% faster with 2).
 // A chunk costs a partial image of traffic and a fixed per-block set-up of about 16
 // views' work, so chunks keep at least 16 views of each axis and there are at most 12.
-int adj_chunks(const ctlib_plan* p, int64_t B) {
+int adj_chunks(const ctlib_plan* p, const Tuning& T, int64_t B) {
     const DevPlan& P = p->dev;
     int vmin = 1 << 30;
     if (P.nx) vmin = P.nx;
     if (P.ny && P.ny < vmin) vmin = P.ny;
     if (vmin == 1 << 30) return 1;
-    if (p->adj_nch > 0) return p->adj_nch < vmin ? p->adj_nch : vmin;
+    if (T.adj_nch > 0) return T.adj_nch < vmin ? T.adj_nch : vmin;
     const int L = adj_tile_lines(P);
     const int64_t items = B * ((P.nx ? (P.H + L - 1) / L : 0) + (P.ny ? (P.W + L - 1) / L : 0));
     int64_t nch = (4096 + items - 1) / items;
@@ -125,14 +138,14 @@ int adj_chunks(const ctlib_plan* p, int64_t B) {
 // adj_chunks: a (image, group of G views) work item's image lines are split so the
 // launch has about 4096 blocks, each chunk writing a partial sinogram that is summed
 // afterwards; chunks keep at least 32 lines of each axis and there are at most 16.
-int fwd_chunks(const ctlib_plan* p, int64_t B, bool weighted) {
+int fwd_chunks(const ctlib_plan* p, const Tuning& T, int64_t B, bool weighted) {
     const DevPlan& P = p->dev;
     int lmin = 1 << 30;
     if (P.nx) lmin = P.H;
     if (P.ny && P.W < lmin) lmin = P.W;
     if (lmin == 1 << 30) return 1;
-    if (p->fwd_nch > 0) return p->fwd_nch < lmin ? p->fwd_nch : lmin;
-    const int G = weighted ? 4 : ((P.D + 1 + 63) / 64 <= 12 && p->fwd_G != 4 ? 8 : 4);
+    if (T.fwd_nch > 0) return T.fwd_nch < lmin ? T.fwd_nch : lmin;
+    const int G = weighted ? 4 : ((P.D + 1 + 63) / 64 <= 12 && T.fwd_G != 4 ? 8 : 4);
     const int64_t items = B * ((P.nx + G - 1) / G + (P.ny + G - 1) / G);
     int64_t nch = (4096 + items - 1) / items;
     if (nch > 16) nch = 16;
@@ -146,20 +159,22 @@ struct WsLayout {
     size_t n_ylines = 0, n_tables = 0, n_yt = 0, n_xp = 0, n_sp = 0, n_rows = 0;
     int nch = 1;   // view chunks of the bin -> pixel launch (partials in xp / yt when > 1)
     int fnch = 1;  // line chunks of the line -> bin launch (partial sinograms in sp when > 1)
+    int fwd_G = 0; // views per line -> bin block override (Tuning::fwd_G)
 };
 
-WsLayout layout(const ctlib_plan* p, int op, int64_t B, int dtype) {
+WsLayout layout(const ctlib_plan* p, const Tuning& T, int op, int64_t B, int dtype) {
     const DevPlan& P = p->dev;
     const size_t es = dsize(dtype);
     const size_t img = (size_t)B * P.H * P.W;
     const size_t sino = (size_t)B * P.V * P.D;
     WsLayout L;
+    L.fwd_G = T.fwd_G;
     auto sino_partials = [&](bool weighted) {
-        L.fnch = fwd_chunks(p, B, weighted);
+        L.fnch = fwd_chunks(p, T, B, weighted);
         L.n_sp = L.fnch > 1 ? L.fnch * sino * es : 0;
     };
     auto partials = [&]() {
-        L.nch = adj_chunks(p, B);
+        L.nch = adj_chunks(p, T, B);
         L.n_yt = P.ny ? L.nch * img * es : 0;
         L.n_xp = (L.nch > 1 && P.nx) ? L.nch * img * es : 0;
     };
@@ -264,16 +279,19 @@ double sweep_bytes(const DevPlan& P, int B, int dt) {
     return es * B * (double)P.V * ((double)P.H * P.W + P.D);
 }
 
-int check_run_args(const ctlib_plan* p, int op, int dtype, const void* in, void* out, int64_t B, void* ws,
-                   size_t ws_bytes) {
+int check_run_args(const ctlib_plan* p, int op, int dtype, const void* in, void* out, int64_t B) {
     if (!p) return fail(CTLIB_ERR_INVALID_ARG, "plan is NULL");
     if (op < CTLIB_OP_PRJ || op > CTLIB_OP_BPRJ_SV) return fail(CTLIB_ERR_INVALID_ARG, "unknown op code");
     if (dtype != CTLIB_F32 && dtype != CTLIB_F64) return fail(CTLIB_ERR_UNSUPPORTED_DTYPE, "dtype must be f32 or f64");
     if (B < 0 || B > (1 << 24)) return fail(CTLIB_ERR_INVALID_ARG, "batch out of range");
     if (B > 0 && (!in || !out)) return fail(CTLIB_ERR_INVALID_ARG, "in/out must be device pointers");
-    const size_t need = layout(p, op, B, dtype).total;
-    if (B > 0 && need > 0 && (!ws || ws_bytes < need))
-        return fail(CTLIB_ERR_WORKSPACE, "workspace smaller than ctlib_workspace_bytes()");
+    return CTLIB_OK;
+}
+
+// `need` is the size of the layout the call will launch with (computed once per call).
+int check_ws(size_t need, const void* ws, size_t ws_bytes, const char* what) {
+    if (need > 0 && (!ws || ws_bytes < need))
+        return fail(CTLIB_ERR_WORKSPACE, std::string("workspace smaller than ") + what);
     return CTLIB_OK;
 }
 
@@ -289,7 +307,7 @@ hipError_t run_to_sino(const ctlib_plan* p, int dtype, const void* in, void* out
         e = launch_to_ylines(in, yl, dtype, B, P.H, P.W, s);
         if (e != hipSuccess) return e;
     }
-    FwdShape sh = default_fwd_shape(P, weighted, p->fwd_G);
+    FwdShape sh = default_fwd_shape(P, weighted, L.fwd_G);
     sh.nch = L.fnch;
     void* dst = L.fnch > 1 ? ws + L.sp : out;
     {
@@ -485,7 +503,7 @@ void ctlib_plan_destroy(ctlib_plan* p) {
 
 size_t ctlib_workspace_bytes(const ctlib_plan* p, int op, int64_t batch, int dtype) {
     if (!p || batch <= 0) return 0;
-    return layout(p, op, batch, dtype).total;
+    return layout(p, snap(p), op, batch, dtype).total;
 }
 
 int ctlib_plan_set_tuning(ctlib_plan* p, int fwd_views_per_block) {
@@ -569,9 +587,12 @@ int ctlib_plan_export(const ctlib_geom* geom, int table, void* dst, size_t* byte
 
 int ctlib_run(const ctlib_plan* p, int op, int dtype, const void* in, void* out, int64_t batch, void* ws,
               size_t ws_bytes, void* stream) {
-    int rc = check_run_args(p, op, dtype, in, out, batch, ws, ws_bytes);
+    int rc = check_run_args(p, op, dtype, in, out, batch);
     if (rc) return rc;
     if (batch == 0) return CTLIB_OK;
+    const WsLayout L = layout(p, snap(p), op, batch, dtype);
+    rc = check_ws(L.total, ws, ws_bytes, "ctlib_workspace_bytes()");
+    if (rc) return rc;
     DeviceGuard guard(p->device);
     if (!guard.ok) return fail(CTLIB_ERR_HIP, "cannot select the plan's device");
     TraceRange range(op_range_name(op));
@@ -579,7 +600,6 @@ int ctlib_run(const ctlib_plan* p, int op, int dtype, const void* in, void* out,
     const int B = (int)batch;
     const int dt = dtype == CTLIB_F64 ? DT_F64 : DT_F32;
     hipStream_t s = static_cast<hipStream_t>(stream);
-    const WsLayout L = layout(p, op, batch, dtype);
     char* w = static_cast<char*>(ws);
     double2* tabs = reinterpret_cast<double2*>(w + L.tables);
     hipError_t e = hipSuccess;
@@ -685,12 +705,33 @@ int ctlib_timing_collect(ctlib_kernel_stats* out) {
     return rc;
 }
 
+}  // extern "C"
+
+namespace {
+
+// Scratch of the residual step: [Y-line copy | Y-partial image] (the layouts of projection
+// and projection_t, overlapping) followed by the bin tables.
+struct ResidLayout {
+    WsLayout la, lb;
+    size_t tables = 0, total = 0;
+};
+
+ResidLayout resid_layout(const ctlib_plan* p, const Tuning& T, int64_t batch, int dtype) {
+    ResidLayout R;
+    R.la = layout(p, T, CTLIB_OP_PRJ, batch, dtype);
+    R.lb = layout(p, T, CTLIB_OP_PRJ_T, batch, dtype);
+    R.tables = R.la.total > R.lb.total ? R.la.total : R.lb.total;
+    R.total = R.tables + align_up((size_t)batch * p->dev.V * (p->dev.D + 2) * sizeof(double2));
+    return R;
+}
+
+}  // namespace
+
+extern "C" {
+
 size_t ctlib_residual_workspace_bytes(const ctlib_plan* p, int64_t batch, int dtype) {
     if (!p || batch <= 0) return 0;
-    const WsLayout a = layout(p, CTLIB_OP_PRJ, batch, dtype);
-    const WsLayout b = layout(p, CTLIB_OP_PRJ_T, batch, dtype);
-    const size_t tables = align_up((size_t)batch * p->dev.V * (p->dev.D + 2) * sizeof(double2));
-    return (a.total > b.total ? a.total : b.total) + tables;
+    return resid_layout(p, snap(p), batch, dtype).total;
 }
 
 int ctlib_run_residual(const ctlib_plan* p, int dtype, const void* x, const void* m, void* y, void* grad,
@@ -700,12 +741,13 @@ int ctlib_run_residual(const ctlib_plan* p, int dtype, const void* x, const void
 
 int ctlib_run_residual_loss(const ctlib_plan* p, int dtype, const void* x, const void* m, void* y, void* grad,
                             double* row_sq, int64_t batch, void* ws, size_t ws_bytes, void* stream) {
-    int rc = check_run_args(p, CTLIB_OP_PRJ_T, dtype, x, grad, batch, ws, ws_bytes);
+    int rc = check_run_args(p, CTLIB_OP_PRJ_T, dtype, x, grad, batch);
     if (rc) return rc;
     if (batch == 0) return CTLIB_OK;
     if (!m) return fail(CTLIB_ERR_INVALID_ARG, "measurement pointer is NULL");
-    if (ws_bytes < ctlib_residual_workspace_bytes(p, batch, dtype))
-        return fail(CTLIB_ERR_WORKSPACE, "workspace smaller than ctlib_residual_workspace_bytes()");
+    const ResidLayout R = resid_layout(p, snap(p), batch, dtype);
+    rc = check_ws(R.total, ws, ws_bytes, "ctlib_residual_workspace_bytes()");
+    if (rc) return rc;
     DeviceGuard guard(p->device);
     if (!guard.ok) return fail(CTLIB_ERR_HIP, "cannot select the plan's device");
     TraceRange range("ctlib:residual_step");
@@ -714,11 +756,9 @@ int ctlib_run_residual_loss(const ctlib_plan* p, int dtype, const void* x, const
     const int dt = dtype == CTLIB_F64 ? DT_F64 : DT_F32;
     hipStream_t s = static_cast<hipStream_t>(stream);
     char* w = static_cast<char*>(ws);
-    // ws: [Y-line copy | Y-partial image] (layout of projection / projection_t) followed by the bin tables
-    const WsLayout la = layout(p, CTLIB_OP_PRJ, batch, dtype);
-    const WsLayout lb = layout(p, CTLIB_OP_PRJ_T, batch, dtype);
-    const size_t base = la.total > lb.total ? la.total : lb.total;
-    double2* tabs = reinterpret_cast<double2*>(w + base);
+    const WsLayout& la = R.la;
+    const WsLayout& lb = R.lb;
+    double2* tabs = reinterpret_cast<double2*>(w + R.tables);
     void* yl = w + la.ylines;
     hipError_t e = hipSuccess;
     if (la.fnch > 1) {
@@ -739,7 +779,7 @@ int ctlib_run_residual_loss(const ctlib_plan* p, int dtype, const void* x, const
     }
     if (e == hipSuccess) {
         KernelTimer t(K_FWD, sweep_bytes(P, B, dt), 0, s);
-        e = launch_fwd_resid(P, dt, x, P.ny ? yl : nullptr, y, m, tabs, B, default_fwd_shape(P, false, p->fwd_G), s,
+        e = launch_fwd_resid(P, dt, x, P.ny ? yl : nullptr, y, m, tabs, B, default_fwd_shape(P, false, la.fwd_G), s,
                              row_sq);
     }
     if (e == hipSuccess) e = run_to_image(p, dt, tabs, grad, B, false, 1.0, lb, w, s);

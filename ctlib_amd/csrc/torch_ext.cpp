@@ -37,8 +37,20 @@ struct Cached {
     ctlib_geom geom;
 };
 
+// Plan cache: at most kMaxPlans geometries per process, least recently used evicted first.
+// A plan used while its stream is being captured (torch.cuda.graph, torch.compile's graph
+// trees) has its device tables' addresses baked into the captured kernels, so it is pinned:
+// kept alive until the process ends, whatever the cache later evicts or clears.
+struct Slot {
+    Cached c;
+    uint64_t last_use = 0;
+};
+
 std::mutex g_mu;
-std::map<std::string, Cached> g_cache;
+std::map<std::string, Slot> g_cache;
+std::vector<PlanPtr> g_pinned;
+uint64_t g_tick = 0;
+int64_t g_created = 0;
 constexpr size_t kMaxPlans = 64;
 
 [[noreturn]] void raise_status(int rc, const char* what) {
@@ -68,14 +80,33 @@ Cached get_plan(const torch::Tensor& options, int device) {
     key.append(reinterpret_cast<const char*>(&device), sizeof(device));
     std::lock_guard<std::mutex> lock(g_mu);
     auto it = g_cache.find(key);
-    if (it != g_cache.end()) return it->second;
+    if (it != g_cache.end()) {
+        it->second.last_use = ++g_tick;
+        return it->second.c;
+    }
     ctlib_plan* raw = nullptr;
     rc = ctlib_plan_create(&g, device, &raw);
     if (rc) raise_status(rc, "plan");
-    if (g_cache.size() >= kMaxPlans) g_cache.clear();
-    Cached c{PlanPtr(raw, PlanDeleter()), g};
-    g_cache.emplace(key, c);
-    return c;
+    g_created++;
+    if (g_cache.size() >= kMaxPlans) {
+        auto lru = g_cache.begin();
+        for (auto j = g_cache.begin(); j != g_cache.end(); ++j)
+            if (j->second.last_use < lru->second.last_use) lru = j;
+        g_cache.erase(lru);  // a pinned plan stays alive through g_pinned
+    }
+    Slot slot{Cached{PlanPtr(raw, PlanDeleter()), g}, ++g_tick};
+    g_cache.emplace(key, slot);
+    return slot.c;
+}
+
+// Pins `plan` for the life of the process when `s` is capturing a graph.
+void pin_if_capturing(const PlanPtr& plan, hipStream_t s) {
+    hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(s, &st) != hipSuccess || st == hipStreamCaptureStatusNone) return;
+    std::lock_guard<std::mutex> lock(g_mu);
+    for (const PlanPtr& q : g_pinned)
+        if (q.get() == plan.get()) return;
+    g_pinned.push_back(plan);
 }
 
 int dtype_code(const torch::Tensor& t) {
@@ -123,6 +154,7 @@ torch::Tensor run_op(const torch::Tensor& in_, const torch::Tensor& options, int
     torch::Tensor ws;
     if (wsb) ws = torch::empty({(int64_t)wsb}, in.options().dtype(torch::kUInt8));
     hipStream_t s = c10::hip::getCurrentHIPStream((c10::DeviceIndex)device).stream();
+    pin_if_capturing(c.plan, s);
     int rc = ctlib_run(c.plan.get(), op, dt, in.data_ptr(), out.data_ptr(), B, wsb ? ws.data_ptr() : nullptr, wsb,
                        (void*)s);
     if (rc) raise_status(rc, name);
@@ -222,6 +254,7 @@ std::vector<torch::Tensor> residual_common(torch::Tensor image, torch::Tensor me
     const size_t wsb = ctlib_residual_workspace_bytes(c.plan.get(), B, dt);
     torch::Tensor ws = torch::empty({(int64_t)wsb}, x.options().dtype(torch::kUInt8));
     hipStream_t s = c10::hip::getCurrentHIPStream((c10::DeviceIndex)device).stream();
+    pin_if_capturing(c.plan, s);
     int rc = ctlib_run_residual_loss(c.plan.get(), dt, x.data_ptr(), m.data_ptr(),
                                      with_loss ? nullptr : first.data_ptr(), grad.data_ptr(),
                                      with_loss ? first.data_ptr<double>() : nullptr, B, ws.data_ptr(), wsb, (void*)s);
@@ -380,9 +413,21 @@ class Batch {
     ctlib_batch* b_ = nullptr;
 };
 
+// Drops every cached plan; plans pinned by a graph capture stay alive (they are no longer
+// looked up, a later call builds a fresh plan).
 void clear_plan_cache() {
     std::lock_guard<std::mutex> lock(g_mu);
     g_cache.clear();
+}
+
+pybind11::dict plan_cache_info() {
+    std::lock_guard<std::mutex> lock(g_mu);
+    pybind11::dict d;
+    d["entries"] = (int64_t)g_cache.size();
+    d["pinned"] = (int64_t)g_pinned.size();
+    d["created"] = g_created;
+    d["capacity"] = (int64_t)kMaxPlans;
+    return d;
 }
 
 }  // namespace
@@ -403,7 +448,8 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m) {
     m.def("laplacian", &laplacian, "kNN graph weights as a sparse (N, N) tensor (HIP)");
     m.def("parse_options", &parse_options, "Parse an options vector");
     m.def("workspace_bytes", &workspace_bytes, "Scratch bytes of one op");
-    m.def("clear_plan_cache", &clear_plan_cache, "Drop all cached geometry plans");
+    m.def("clear_plan_cache", &clear_plan_cache, "Drop all cached geometry plans (graph-pinned plans stay alive)");
+    m.def("plan_cache_info", &plan_cache_info, "Cached / pinned / created plan counts and the cache capacity");
     pybind11::class_<Batch>(m, "Batch")
         .def(pybind11::init<std::vector<torch::Tensor>, int64_t, bool>())
         .def("workspace_bytes", &Batch::workspace_bytes)

@@ -59,3 +59,17 @@ def test_public_surface_matches_the_reference_module():
     for name in ("projection", "projection_t", "backprojection", "backprojection_t", "backprojection_sv", "fbp",
                  "laplacian"):
         assert callable(getattr(ctlib_amd, name)), name
+
+
+def test_reference_module_name_imports():
+    """`import ctlib` (the reference's module name, setup.py:9) exposes exactly the seven functions of
+    ctlib.cpp:106-114, each the library's own entry point, and refuses host tensors like the reference."""
+    import ctlib
+
+    names = ("projection", "projection_t", "backprojection_t", "backprojection", "backprojection_sv", "fbp",
+             "laplacian")
+    assert sorted(ctlib.__all__) == sorted(names)
+    for name in names:
+        assert getattr(ctlib, name) is getattr(ctlib_amd, name), name
+    with pytest.raises(RuntimeError, match="GPU"):
+        ctlib.projection(torch.zeros(1, 1, 10, 12), O_FAN)

@@ -12,7 +12,9 @@ fixes:
     column / row sums; parallel-beam mass conservation;
   * invariance to the 256-bin detector chunking of the reference's grid;
   * textbook properties of the RL taps and of the conv2d cross-correlation;
-  * FBP of a uniform disk reconstructs the disk.
+  * FBP of a uniform disk reconstructs the disk (all three geometries), and the
+    fbp pre-weights equal the cosine of each bin-centre ray measured from the
+    ray geometry itself.
 Each check is chosen so that a dropped factor, a wrong sign or index, a
 swapped axis or a reversed bin order fails it.
 """
@@ -202,9 +204,12 @@ def test_ramp_is_cross_correlation():
         for r in range(3):
             full = np.correlate(y[r] * pw, h, mode="full")  # c[k] = sum_n a[n+k-(N-1)] * h[n]
             D = g["dets"]
-            ref = np.array([np.sum(y[r] * pw * h[np.arange(D) - j + D - 1]) for j in range(D)])
-            np.testing.assert_allclose(q[r], ref, rtol=1e-12, atol=1e-12 * np.abs(ref).max())
             assert full.shape[0] == 3 * D - 2
+            # q[j] = sum_m a[m] h[m - j + D - 1] = c[k] at lag k = j - (D - 1), i.e. full[j + D - 1]
+            np.testing.assert_allclose(q[r], full[D - 1:2 * D - 1], rtol=1e-12, atol=1e-12 * np.abs(full).max())
+            # the same from the definition of a zero-padded convolution with the flipped taps
+            conv = np.convolve(y[r] * pw, h[::-1], mode="full")
+            np.testing.assert_allclose(q[r], conv[D - 1:2 * D - 1], rtol=1e-12, atol=1e-12 * np.abs(conv).max())
 
 
 def test_preweights():
@@ -220,12 +225,80 @@ def test_preweights():
     np.testing.assert_allclose(pw, 3.0 * 0.04 * np.cos((np.arange(8) - 3.5) * 0.04), rtol=1e-14)
 
 
-@pytest.mark.parametrize("st", [0, 2])
+def _bin_centre_angles(o, v):
+    """Signed angle between the central ray (source -> isocentre) and the ray through the centre of
+    every bin of view v (output bin order), from the oracle's axis-mapped bin edges and the source
+    position: each edge point on the sweep axis gives an edge ray, and a fan-EA bin centre lies half
+    way in angle between its two edge rays."""
+    g = O.parse(o)
+    xmode, rev, e, _ = O.view(o, v)
+    th = v * g["dAng"] + g["Ang0"]
+    S = np.array([-math.sin(th), math.cos(th)]) * g["s2r"]
+    c = -S
+    P = np.stack([e, np.zeros_like(e)], 1) if xmode else np.stack([np.zeros_like(e), e], 1)
+    r = P - S
+    a = np.arctan2(c[0] * r[:, 1] - c[1] * r[:, 0], r @ c)
+    mid = 0.5 * (a[1:] + a[:-1])
+    return mid[::-1] if rev else mid
+
+
+def test_ea_preweight_geometry():
+    """fan_ea_kernel.cu:1075-1080: the EA fbp pre-weight is s2r * dDet * cos(gamma_j), gamma_j the angle
+    of bin j's centre ray off the central ray.  gamma_j is measured here from the ray geometry (source
+    position and the oracle's mapped bin edges), not from the (j - D/2 + 1/2) dDet formula, and the
+    angles themselves must be D/2 - 1/2 bins apart from the centre on both sides (no half-bin shift)."""
+    for D, dDet, s2r in ((17, 0.03, 3.0), (24, 0.011, 5.95), (31, 0.05, 4.0)):
+        o = fan(1, V=12, D=D, W=10, H=10, dImg=0.1, dDet=dDet, Ang0=0.3, s2r=s2r, d2r=2.0, bs=0.0)
+        pw, delta = O.fbp_preweight(o)
+        assert delta == pytest.approx(dDet, rel=1e-15)
+        for v in range(12):
+            gam = _bin_centre_angles(o, v)
+            np.testing.assert_allclose(np.abs(gam), np.abs((np.arange(D) - D / 2 + 0.5) * dDet), rtol=0, atol=1e-12)
+            np.testing.assert_allclose(pw, s2r * dDet * np.cos(gam), rtol=1e-12)
+        # the reference's pre-weight ignores binshift (fan_ea_kernel.cu:1075): same values with a shift
+        o2 = o.copy()
+        o2[10] = 0.7 * dDet
+        np.testing.assert_array_equal(O.fbp_preweight(o2)[0], pw)
+
+
+def test_ed_preweight_geometry():
+    """fan_ed_kernel.cu:1349-1355: the ED pre-weight is vd * cos of the angle between bin j's centre ray
+    (through the midpoint of its edges on the virtual detector at the isocentre) and the central ray.
+    The detector point of each edge is recovered by intersecting the edge ray (source -> mapped axis
+    edge) with the detector line, not from the (j - D/2 + 1/2) vd formula."""
+    o = fan(0, V=10, D=21, W=10, H=10, dImg=0.1, dDet=0.13, Ang0=0.2, s2r=3.0, d2r=2.0, bs=0.0)
+    g = O.parse(o)
+    pw, vd = O.fbp_preweight(o)
+    assert vd == pytest.approx(0.13 * 3.0 / 5.0, rel=1e-15)
+    for v in range(10):
+        xmode, rev, e, _ = O.view(o, v)
+        th = v * g["dAng"] + g["Ang0"]
+        S = np.array([-math.sin(th), math.cos(th)]) * 3.0
+        u_dir = np.array([math.cos(th), math.sin(th)])
+        P = np.stack([e, np.zeros_like(e)], 1) if xmode else np.stack([np.zeros_like(e), e], 1)
+        # S + lam (P - S) on the detector line {u * u_dir} (perpendicular to S): (S + lam (P - S)) . S = 0
+        lam = np.array([(S @ S) / ((S - p) @ S) for p in P])
+        u = np.array([(S + l_ * (p - S)) @ u_dir for l_, p in zip(lam, P)])
+        mid = 0.5 * (u[1:] + u[:-1])
+        mid = mid[::-1] if rev else mid
+        pc = mid[:, None] * u_dir[None, :] - S[None, :]
+        cosang = (pc @ (-S)) / (np.linalg.norm(pc, axis=1) * np.linalg.norm(S))
+        np.testing.assert_allclose(pw, vd * cosang, rtol=1e-12)
+
+
+@pytest.mark.parametrize("st", [0, 1, 2])
 def test_fbp_uniform_disk(st):
-    """FBP of the projections of a uniform disk recovers the disk value in its interior."""
+    """FBP of the projections of a uniform disk recovers the disk value in its interior.
+
+    Fan-EA (fan_ea_kernel.cu:1059-1107) uses the plain RL taps without the equal-angle factor
+    (gamma / sin gamma)^2, so the fan is kept narrow (|gamma| <= 0.15 rad: the missing factor is at
+    most 1.0075) while still covering the whole image."""
     n = 64
     if st == 2:
         o = para(V=180, D=96, W=n, H=n, dImg=1 / 32, dDet=1 / 32, Ang0=0.0, dAng=math.pi / 180, bs=0.0)
+    elif st == 1:
+        o = fan(1, V=360, D=128, W=n, H=n, dImg=1 / 32, dDet=0.3 / 128, Ang0=0.0, dAng=2 * math.pi / 360,
+                s2r=10.0, d2r=5.0, bs=0.0)
     else:
         o = fan(0, V=360, D=128, W=n, H=n, dImg=1 / 32, dDet=1 / 32, Ang0=0.0, dAng=2 * math.pi / 360,
                 s2r=6.0, d2r=3.0, bs=0.0)
