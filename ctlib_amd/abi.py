@@ -25,7 +25,7 @@ EXPORTS = ("ctlib_parse_options", "ctlib_plan_create", "ctlib_plan_destroy", "ct
            "ctlib_status_string", "ctlib_batch_create", "ctlib_batch_destroy", "ctlib_batch_workspace_bytes",
            "ctlib_batch_set_schedule", "ctlib_batch_bind", "ctlib_batch_run", "ctlib_batch_info",
            "ctlib_batch_last_error", "ctlib_plan_set_tuning", "ctlib_plan_set_chunks", "ctlib_batch_check",
-           "ctlib_laplacian", "ctlib_laplacian_workspace_bytes")
+           "ctlib_laplacian", "ctlib_laplacian_workspace_bytes", "ctlib_run_noalloc")
 
 
 class Geom(ctypes.Structure):

@@ -94,6 +94,39 @@ Tuning snap(const ctlib_plan* p) {
                   p->fwd_nch.load(std::memory_order_relaxed)};
 }
 
+// ---- scratch of ctlib_run_noalloc: one fixed device array per GPU, no allocation per call.
+// `scratch` names the host-side stand-in of the array; its device address on a GPU comes
+// from hipGetSymbolAddress with that GPU current (resolve_scratch, called when a plan is
+// created on the device) and is the only form in which host code passes it on.
+constexpr size_t kScratchBytes = CTLIB_NOALLOC_SCRATCH_BYTES;
+constexpr int kMaxDevices = 64;
+std::mutex g_scr_mu;
+void* g_scr_ptr[kMaxDevices] = {};
+bool g_scr_tried[kMaxDevices] = {};
+
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[CTLIB_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+
+// With `device` current: records this device's address of `scratch` (once per device).
+void resolve_scratch(int device) {
+    if (device < 0 || device >= kMaxDevices) return;
+    std::lock_guard<std::mutex> lock(g_scr_mu);
+    if (g_scr_tried[device]) return;
+    g_scr_tried[device] = true;
+    void* ptr = nullptr;
+    if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) == hipSuccess) g_scr_ptr[device] = ptr;
+    else (void)hipGetLastError();
+}
+
+void* scratch_ptr(int device) {
+    if (device < 0 || device >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lock(g_scr_mu);
+    return g_scr_ptr[device];
+}
+
 inline size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
 
 size_t dsize(int dtype) { return dtype == CTLIB_F64 ? sizeof(double) : sizeof(float); }
@@ -249,9 +282,18 @@ hipEvent_t pool_get() {
     return e;
 }
 
+// Set while the calling thread runs ctlib_run_noalloc: its launches record no events
+// (event creation may allocate).
+thread_local bool t_no_timing = false;
+struct NoTiming {
+    NoTiming() { t_no_timing = true; }
+    ~NoTiming() { t_no_timing = false; }
+};
+
 class KernelTimer {
   public:
     KernelTimer(int cls, double bytes, double flops, hipStream_t s) : s_(s) {
+        if (t_no_timing) return;
         std::lock_guard<std::mutex> lock(g_tmu);
         if (!g_timing) return;
         t_ = TimedLaunch{pool_get(), pool_get(), cls, bytes, flops};
@@ -350,6 +392,67 @@ hipError_t run_ramp(const ctlib_plan* p, int in_dt, const void* in, int out_dt, 
     const long rows = (long)B * P.V;
     KernelTimer t(K_RAMP, 0, 2.0 * rows * (double)P.D * P.D, s);
     return launch_ramp(P, in_dt, in, out_dt, out, rows, pre, post, scale, s);
+}
+
+// One operator with the scratch layout L at w (ctlib_run and ctlib_run_noalloc).
+int run_with(const ctlib_plan* p, int op, int dtype, const void* in, void* out, int64_t batch, const WsLayout& L,
+             char* w, hipStream_t s) {
+    DeviceGuard guard(p->device);
+    if (!guard.ok) return fail(CTLIB_ERR_HIP, "cannot select the plan's device");
+    TraceRange range(op_range_name(op));
+    const DevPlan& P = p->dev;
+    const int B = (int)batch;
+    const int dt = dtype == CTLIB_F64 ? DT_F64 : DT_F32;
+    double2* tabs = reinterpret_cast<double2*>(w + L.tables);
+    hipError_t e = hipSuccess;
+    switch (op) {
+    case CTLIB_OP_PRJ:
+        e = run_to_sino(p, dt, in, out, B, false, P.post_prj, L, w, s);
+        break;
+    case CTLIB_OP_BPRJ_T:
+        e = run_to_sino(p, dt, in, out, B, true, P.scan_type == SCAN_FAN_EA ? P.post_bt : nullptr, L, w, s);
+        break;
+    case CTLIB_OP_BPRJ_T_TRUE:
+        e = run_to_sino(p, dt, in, out, B, true, nullptr, L, w, s);
+        break;
+    case CTLIB_OP_PRJ_T:
+        e = run_tables(p, dt, in, nullptr, P.post_prj, tabs, B, s);
+        if (e == hipSuccess) e = run_to_image(p, dt, tabs, out, B, false, 1.0, L, w, s);
+        break;
+    case CTLIB_OP_BPRJ:
+        e = run_tables(p, dt, in, nullptr, nullptr, tabs, B, s);
+        if (e == hipSuccess) e = run_to_image(p, dt, tabs, out, B, true, 1.0, L, w, s);
+        break;
+    case CTLIB_OP_FBP: {
+        // pre-weight + RL ramp (Toeplitz GEMM) -> weighted backprojection * dAng/2 (fan_ed_kernel.cu:1349-1380)
+        double* rows = reinterpret_cast<double*>(w + L.rows);
+        e = run_ramp(p, dt, in, DT_F64, rows, B, P.preweight, nullptr, 1.0, s);
+        if (e == hipSuccess) e = run_tables(p, DT_F64, rows, nullptr, nullptr, tabs, B, s);
+        if (e == hipSuccess) e = run_to_image(p, dt, tabs, out, B, true, P.dAng / 2, L, w, s);
+        break;
+    }
+    case CTLIB_OP_FBP_T: {
+        // adjoint of fbp: exact backprojection adjoint, then ramp * dAng/2, then pre-weight
+        void* rows = w + L.rows;
+        e = run_to_sino(p, dt, in, rows, B, true, nullptr, L, w, s);
+        if (e == hipSuccess) e = run_ramp(p, dt, rows, dt, out, B, nullptr, P.preweight, P.dAng / 2, s);
+        break;
+    }
+    case CTLIB_OP_BPRJ_SV: {
+        // per-view terms of backprojection, one image per (b, view) (fan_ed_kernel.cu:1299-1331)
+        if ((size_t)B * P.V > (size_t)INT32_MAX) return fail(CTLIB_ERR_INVALID_ARG, "batch * views exceeds 2^31");
+        e = run_tables(p, dt, in, nullptr, nullptr, tabs, B, s);
+        if (e == hipSuccess) {
+            KernelTimer t(K_SV, sweep_bytes(P, B, dt), 0, s);
+            e = launch_bprj_sv(P, dt, tabs, out, B, 1.0, s);
+        }
+        break;
+    }
+    default:
+        return fail(CTLIB_ERR_INVALID_ARG, "unknown op code");
+    }
+    if (e != hipSuccess) return hip_fail(e, "kernel launch");
+    return CTLIB_OK;
 }
 
 }  // namespace
@@ -465,6 +568,7 @@ int ctlib_plan_create(const ctlib_geom* geom, int device, ctlib_plan** out) {
             return hip_fail(e, "hipMemcpy(plan)");
         }
     }
+    resolve_scratch(device);
     DevPlan& P = p->dev;
     P.vp = reinterpret_cast<const ViewParams*>(base + o_vp);
     P.edges = reinterpret_cast<const double*>(base + o_edges);
@@ -593,64 +697,32 @@ int ctlib_run(const ctlib_plan* p, int op, int dtype, const void* in, void* out,
     const WsLayout L = layout(p, snap(p), op, batch, dtype);
     rc = check_ws(L.total, ws, ws_bytes, "ctlib_workspace_bytes()");
     if (rc) return rc;
-    DeviceGuard guard(p->device);
-    if (!guard.ok) return fail(CTLIB_ERR_HIP, "cannot select the plan's device");
-    TraceRange range(op_range_name(op));
-    const DevPlan& P = p->dev;
-    const int B = (int)batch;
-    const int dt = dtype == CTLIB_F64 ? DT_F64 : DT_F32;
-    hipStream_t s = static_cast<hipStream_t>(stream);
-    char* w = static_cast<char*>(ws);
-    double2* tabs = reinterpret_cast<double2*>(w + L.tables);
-    hipError_t e = hipSuccess;
-    switch (op) {
-    case CTLIB_OP_PRJ:
-        e = run_to_sino(p, dt, in, out, B, false, P.post_prj, L, w, s);
-        break;
-    case CTLIB_OP_BPRJ_T:
-        e = run_to_sino(p, dt, in, out, B, true, P.scan_type == SCAN_FAN_EA ? P.post_bt : nullptr, L, w, s);
-        break;
-    case CTLIB_OP_BPRJ_T_TRUE:
-        e = run_to_sino(p, dt, in, out, B, true, nullptr, L, w, s);
-        break;
-    case CTLIB_OP_PRJ_T:
-        e = run_tables(p, dt, in, nullptr, P.post_prj, tabs, B, s);
-        if (e == hipSuccess) e = run_to_image(p, dt, tabs, out, B, false, 1.0, L, w, s);
-        break;
-    case CTLIB_OP_BPRJ:
-        e = run_tables(p, dt, in, nullptr, nullptr, tabs, B, s);
-        if (e == hipSuccess) e = run_to_image(p, dt, tabs, out, B, true, 1.0, L, w, s);
-        break;
-    case CTLIB_OP_FBP: {
-        // pre-weight + RL ramp (Toeplitz GEMM) -> weighted backprojection * dAng/2 (fan_ed_kernel.cu:1349-1380)
-        double* rows = reinterpret_cast<double*>(w + L.rows);
-        e = run_ramp(p, dt, in, DT_F64, rows, B, P.preweight, nullptr, 1.0, s);
-        if (e == hipSuccess) e = run_tables(p, DT_F64, rows, nullptr, nullptr, tabs, B, s);
-        if (e == hipSuccess) e = run_to_image(p, dt, tabs, out, B, true, P.dAng / 2, L, w, s);
-        break;
+    return run_with(p, op, dtype, in, out, batch, L, static_cast<char*>(ws), static_cast<hipStream_t>(stream));
+}
+
+int ctlib_run_noalloc(const ctlib_plan* p, int op, int dtype, const void* in, void* out, int64_t batch,
+                      void* stream) {
+    int rc = check_run_args(p, op, dtype, in, out, batch);
+    if (rc) return rc;
+    if (batch == 0) return CTLIB_OK;
+    // the automatic launch split when its partials fit the scratch array, else fewer chunks
+    Tuning T = snap(p);
+    WsLayout L = layout(p, T, op, batch, dtype);
+    while (L.total > kScratchBytes && (L.fnch > 1 || L.nch > 1)) {
+        T.fwd_nch = L.fnch > 1 ? L.fnch / 2 : 1;
+        T.adj_nch = L.nch > 1 ? L.nch / 2 : 1;
+        L = layout(p, T, op, batch, dtype);
     }
-    case CTLIB_OP_FBP_T: {
-        // adjoint of fbp: exact backprojection adjoint, then ramp * dAng/2, then pre-weight
-        void* rows = w + L.rows;
-        e = run_to_sino(p, dt, in, rows, B, true, nullptr, L, w, s);
-        if (e == hipSuccess) e = run_ramp(p, dt, rows, dt, out, B, nullptr, P.preweight, P.dAng / 2, s);
-        break;
-    }
-    case CTLIB_OP_BPRJ_SV: {
-        // per-view terms of backprojection, one image per (b, view) (fan_ed_kernel.cu:1299-1331)
-        if ((size_t)B * P.V > (size_t)INT32_MAX) return fail(CTLIB_ERR_INVALID_ARG, "batch * views exceeds 2^31");
-        e = run_tables(p, dt, in, nullptr, nullptr, tabs, B, s);
-        if (e == hipSuccess) {
-            KernelTimer t(K_SV, sweep_bytes(P, B, dt), 0, s);
-            e = launch_bprj_sv(P, dt, tabs, out, B, 1.0, s);
-        }
-        break;
-    }
-    default:
-        return fail(CTLIB_ERR_INVALID_ARG, "unknown op code");
-    }
-    if (e != hipSuccess) return hip_fail(e, "kernel launch");
-    return CTLIB_OK;
+    if (L.total > kScratchBytes)
+        return fail(CTLIB_ERR_WORKSPACE, "ctlib_run_noalloc: this op and batch need " + std::to_string(L.total) +
+                                             " bytes of scratch, above its fixed " + std::to_string(kScratchBytes) +
+                                             " (use ctlib_run with a workspace, or a smaller batch)");
+    void* ptr = scratch_ptr(p->device);
+    if (L.total > 0 && !ptr)
+        return fail(CTLIB_ERR_HIP, "ctlib_run_noalloc: the scratch array's address on device " +
+                                       std::to_string(p->device) + " is unknown (hipGetSymbolAddress failed)");
+    NoTiming quiet;
+    return run_with(p, op, dtype, in, out, batch, L, static_cast<char*>(ptr), static_cast<hipStream_t>(stream));
 }
 
 size_t ctlib_laplacian_workspace_bytes(int64_t n, int d, int k, int dtype) {

@@ -88,6 +88,23 @@ size_t ctlib_workspace_bytes(const ctlib_plan* plan, int op, int64_t batch, int 
 int ctlib_run(const ctlib_plan* plan, int op, int dtype, const void* in, void* out, int64_t batch,
               void* ws, size_t ws_bytes, void* stream);
 
+/* Allocation-free variant of ctlib_run, with no counterpart in the reference: for
+ * callers that cannot allocate device memory around a call (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same ops, layouts,
+ * stream semantics and results as ctlib_run, but the scratch comes from one fixed
+ * device array of CTLIB_NOALLOC_SCRATCH_BYTES per GPU inside the library (held by
+ * every GPU the library runs on until the process ends), and the call makes no
+ * hipMalloc / hipFree and no other allocation.  When the automatic launch split of
+ * (op, batch) needs more scratch than that, fewer chunks are used; when even an
+ * unsplit launch does not fit, it returns CTLIB_ERR_WORKSPACE without launching.
+ * Every call on one GPU uses that GPU's array and nothing in it carries over between
+ * calls, so calls that overlap in time on one GPU (another thread or another stream)
+ * race: serialise them.  The array's address on a device is resolved when the first
+ * plan is created there. */
+#define CTLIB_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+int ctlib_run_noalloc(const ctlib_plan* plan, int op, int dtype, const void* in, void* out, int64_t batch,
+                      void* stream);
+
 /* One least-squares data-term step, the unit of work of an unrolled
  * reconstruction:  y = A x  and  grad = A^T (A x - m).  `y` may be NULL.
  * No reference counterpart (the reference has no fused path). */
