@@ -12,11 +12,14 @@ timed on three paths:
     graph     the fused step captured once in a HIP graph and replayed
 `value` is the autograd path's iterations/s.
 
-roofline: the dominant DD kernel's algorithmic (sweep-model) bytes per launch,
-4*B*V*(H*W + D), over its mean launch time from HIP events the library records
-on the launch stream during the timed region; peak 8 TB/s HBM.
-cpu_baseline: the FP64 C++ oracle (oracle/) on the host cores, one batch slice
-of a subset of the views, scaled to the full workload.
+roofline: the dominant DD kernel's VALU issue rate: its VALU wave-instructions per
+launch (rocprofv3 counters, profiles/dd_issue_counters.json) over its mean launch
+time from HIP events the library records on the launch stream during the timed
+region, against 256 CUs x 2.4 GHz x 1 wave-instruction per CU-cycle.  Its HBM
+traffic (profiles/hbm_traffic.json) and the sweep-model equivalent are beside it.
+cpu_baseline: the FP64 C++ oracle (oracle/) on the host cores running the same
+step slice by slice (bounded at 30 s, scaled to the batch); its gradients also
+check the GPU's (oracle_check).
 """
 import argparse
 import json
@@ -44,16 +47,40 @@ def calib_copy():
     del src, dst
 
 
+# MI355X: 256 CUs at 2.4 GHz; one VALU wave-instruction per cycle per CU (four SIMDs, four
+# cycles per wave64 instruction each).
+CUS = 256
+CLOCK_GHZ = 2.4
+VALU_PEAK_GINST = CUS * CLOCK_GHZ
+
+
+def _profile(name):
+    try:
+        with open(os.path.join(ROOT, "profiles", name)) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return None
+
+
 def measured_traffic(kernel):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
     summary (profiles/hbm_traffic.json), scaled by the copy-kernel calibration; None if absent."""
-    path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    data = _profile("hbm_traffic.json")
     try:
-        with open(path) as f:
-            data = json.load(f)
         k = data["kernels"][kernel]
         return float(k["fetch_kib"]) * 1024 * data["scale"]["fetch"] + float(k["write_kib"]) * 1024 * data["scale"]["write"]
-    except (OSError, KeyError, ValueError, TypeError):
+    except (KeyError, ValueError, TypeError):
+        return None
+
+
+def issue_counters(kernel):
+    """Per-launch SQ counters of `kernel` at the metric workload from the committed rocprofv3
+    --pmc summary (profiles/dd_issue_counters.json); None if absent.  The instruction counts of
+    these branch-free kernels depend only on the geometry and batch, not on the data."""
+    data = _profile("dd_issue_counters.json")
+    try:
+        return data["kernels"][kernel]
+    except (KeyError, TypeError):
         return None
 
 
@@ -62,26 +89,49 @@ def workload_options(views=736):
     return phantom.fan_options(views, 736, 512, 512, 0.0066, 0.0072, 5.95, 4.906, 0, dAng=2 * math.pi / views)
 
 
-def cpu_baseline(budget_s=12.0):
-    """FP64 oracle: projection + projection_t of one slice over a view subset, scaled to B=32, V=736."""
-    from ctlib_amd import phantom
+def host_cpu():
+    """(nproc, CPU model) of the host the baseline runs on."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return os.cpu_count(), model
+
+
+def cpu_baseline(x0, m, grad, budget_s=30.0):
+    """FP64 oracle on the host cores: the same step, y = A x0 and g = A^T (y - m), slice by slice
+    on the workload's own fp32 inputs, until every slice is done or `budget_s` has passed; the rate
+    is scaled to the full batch.  The oracle's gradients are also the checker of the GPU's `grad`
+    (relative L2 per slice, the largest reported)."""
     from oracle import oracle as O
-    x = phantom.phantom_batch(1, 512, 512, config=1, scale=0.02)[:, 0].astype(np.float64)
+    o = workload_options().astype(np.float64)
+    B = x0.shape[0]
     threads = O.max_threads()
-    views = 23
-    while True:
-        o = workload_options(views).astype(np.float64)
+    nproc, model = host_cpu()
+    xs = x0[:, 0].double().cpu().numpy()
+    ms = m[:, 0].double().cpu().numpy()
+    gs = grad[:, 0].double().cpu().numpy()
+    t_total, done, worst = 0.0, 0, 0.0
+    for s in range(B):
         t0 = time.perf_counter()
-        y = O.apply("projection", o, x)
-        O.apply("projection_t", o, y)
-        dt = time.perf_counter() - t0
-        if dt * 4 > budget_s or views >= 736:
+        y = O.apply("projection", o, xs[s:s + 1])
+        g = O.apply("projection_t", o, y - ms[s:s + 1])
+        t_total += time.perf_counter() - t0
+        worst = max(worst, float(np.linalg.norm(gs[s] - g[0]) / np.linalg.norm(g[0])))
+        done += 1
+        if t_total > budget_s:
             break
-        views *= 2
-    per_step = dt * (736.0 / views) * 32
+    per_step = t_total / done * B
     return {"value": 1.0 / per_step, "unit": "iters/s", "cores": threads, "kind": "port",
-            "sample": f"FP64 oracle projection+projection_t, 1 slice x {views} of 736 views "
-                      f"({dt:.2f} s wall on {threads} threads), scaled x{736 / views:.0f} views x32 slices"}
+            "nproc": nproc, "cpu_model": model,
+            "sample": f"FP64 oracle y = A x, g = A^T(y - m) on {done} of {B} slices x 736 views "
+                      f"({t_total:.2f} s wall on {threads} OpenMP threads), scaled to {B} slices"}, \
+        {"slices_checked": done, "max_rel_l2_grad_vs_fp64_oracle": worst}
 
 
 def sweep_bytes(g, es=4):
@@ -541,14 +591,35 @@ def main():
     per_launch_ms = st["ms"] / max(st["launches"], 1)
     per_launch_bytes = st["algo_bytes"] / max(st["launches"], 1)
     achieved = per_launch_bytes / (per_launch_ms * 1e-3) / 1e9
-    traffic = measured_traffic("k_fwd" if k == "fwd" else "k_adj") if B == 32 else None
-    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-            "kernel": "k_fwd (line->bin)" if k == "fwd" else "k_adj (bin->pixel)",
-            "kernel_ms": round(per_launch_ms, 4), "algo_bytes_per_launch": per_launch_bytes,
-            # compulsory bytes of one operator application: the image and the sinogram once each
-            "compulsory_bytes_per_launch": 4 * B * (512 * 512 + 736 * 736),
-            "measured_hbm_gbs": round(traffic / (per_launch_ms * 1e-3) / 1e9, 1) if traffic else None}
+    kname = "k_fwd" if k == "fwd" else "k_adj"
+    traffic = measured_traffic(kname) if B == 32 else None
+    ctr = issue_counters(kname) if B == 32 else None
+    # The DD kernels reuse image lines and bin tables from LDS, so what binds them is on chip:
+    # the VALU issue rate (instructions counted by rocprofv3, time measured here) and the LDS.
+    # The HBM figures are reported beside it.
+    roof = {"kernel": "k_fwd (line->bin)" if k == "fwd" else "k_adj (bin->pixel)",
+            "kernel_ms": round(per_launch_ms, 4)}
+    if ctr:
+        valu = ctr["SQ_INSTS_VALU"] / (per_launch_ms * 1e-3) / 1e9
+        roof.update({"bound": "valu", "achieved": round(valu, 1), "peak": VALU_PEAK_GINST,
+                     "unit": "G VALU wave-instructions/s", "frac": round(valu / VALU_PEAK_GINST, 4),
+                     "traffic": traffic,
+                     "counters_source": "profiles/dd_issue_counters.json (rocprofv3 --pmc, per launch)",
+                     "valu_insts_per_launch": ctr["SQ_INSTS_VALU"],
+                     "valu_busy_frac": ctr.get("valu_busy_frac"), "lds_busy_frac": ctr.get("lds_busy_frac"),
+                     "lds_bank_conflict_frac": ctr.get("lds_bank_conflict_frac")})
+    else:
+        roof.update({"bound": "valu", "achieved": None, "peak": VALU_PEAK_GINST,
+                     "unit": "G VALU wave-instructions/s", "frac": None, "traffic": traffic})
+    roof.update({
+        "traffic_source": "profiles/hbm_traffic.json (rocprofv3 FETCH_SIZE / WRITE_SIZE, scaled by a 256 MiB copy)",
+        "measured_hbm_gbs": round(traffic / (per_launch_ms * 1e-3) / 1e9, 1) if traffic else None,
+        "measured_hbm_frac": round(traffic / (per_launch_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4) if traffic else None,
+        # sweep model: every view streams the whole image once plus its sinogram row
+        "sweep_bytes_per_launch": per_launch_bytes,
+        "sweep_equiv_frac_8TBs": round(achieved / HBM_PEAK_GBS, 4),
+        # compulsory bytes of one operator application: the image and the sinogram once each
+        "compulsory_bytes_per_launch": 4 * B * (512 * 512 + 736 * 736)})
     kernels = {c: {"launches": stats[c]["launches"],
                    "ms_per_launch": round(stats[c]["ms"] / max(stats[c]["launches"], 1), 4)}
                for c in ("fwd", "adj", "ramp", "aux")}
@@ -571,13 +642,13 @@ def main():
         "config": {"workload": f"fan-ED 512x512, 736 views x 736 dets, B={B}: y = A x, g = A^T(A x - m)",
                    "global_batch": B, "views": 736, "dets": 736, "image": [512, 512], "scan_type": 0},
         "paths_iters_per_s": {p: round(1000.0 / t, 3) for p, t in results.items()},
-        "step_sweep_pct_hbm": round(step_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS * 100, 2),
+        "step_sweep_equiv_pct_8TBs": round(step_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS * 100, 2),
         "grad_agreement_rel_l2": agree,
         "kernels": kernels,
         "roofline": roof,
     }
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline()
+        out["cpu_baseline"], out["oracle_check"] = cpu_baseline(x0, m, g_ag)
     print(json.dumps(out), flush=True)
 
 

@@ -106,8 +106,8 @@ def laplacian(input, k):
     input: N x d points (float32 or float64, contiguous, on the GPU).  Returns the sparse COO
     (N, N) float32 tensor whose row i holds, at the k points nearest to point i by squared
     distance (itself included), exp(-dist / median) normalised to sum 1, median being the lower
-    median of all N*k selected distances.  The N x N distance matrix is never formed.
-    Supported: d <= 128 and k <= 64 with d + 2k <= 192 (float32) or 2d + 3k <= 192 (float64).
+    median of all N*k selected distances.  The N x N distance matrix is never formed (large d or
+    k compute it in chunks of rows).  Supported: any d, k <= 4096.
     """
     return _C.laplacian(input, int(k))
 

@@ -250,11 +250,12 @@ WsLayout layout(const ctlib_plan* p, const Tuning& T, int op, int64_t B, int dty
     return L;
 }
 
+// Views per block the line -> bin launch will use: two views per wave (8 per block) when the
+// edge slots fit (<= 12 per view) unless tuned to 4, else one (4 per block); weighted: 4.
 FwdShape default_fwd_shape(const DevPlan& P, bool weighted, int tuned = 0) {
     if (weighted) return FwdShape{4, 1};
-    if (tuned == 4 || tuned == 8) return FwdShape{tuned, 1};
-    // two views per wave when the edge slots fit (<= 12 per view), else one
-    return FwdShape{(P.D + 1 + 63) / 64 <= 12 ? 8 : 4, 1};
+    const bool two = (P.D + 1 + 63) / 64 <= 12;
+    return FwdShape{two && tuned != 4 ? 8 : 4, 1};
 }
 
 // ---- kernel timing (HIP event pairs around the main launches, on their stream)
@@ -736,9 +737,7 @@ int ctlib_laplacian(const void* x, int64_t n, int d, int k, int dtype, int64_t* 
     if (n < 1 || n > 0x7fffffff || d < 1) return fail(CTLIB_ERR_INVALID_ARG, "need 1 <= points < 2^31, dimension >= 1");
     if (k < 1 || k > n) return fail(CTLIB_ERR_INVALID_ARG, "k must be in [1, number of points]");
     if (!x || !cols || !vals) return fail(CTLIB_ERR_INVALID_ARG, "NULL pointer");
-    if (!laplacian_supported(d, k, dtype))
-        return fail(CTLIB_ERR_INVALID_ARG, "supported: dimension <= 128, k <= 64, dimension + 2k <= 192 (fp32) or "
-                                           "2 dimension + 3k <= 192 (fp64)");
+    if (!laplacian_supported(d, k, dtype)) return fail(CTLIB_ERR_INVALID_ARG, "supported: k <= 4096");
     if (!ws || ws_bytes < ctlib_laplacian_workspace_bytes(n, d, k, dtype))
         return fail(CTLIB_ERR_WORKSPACE, "workspace smaller than ctlib_laplacian_workspace_bytes()");
     TraceRange range("ctlib:laplacian");
@@ -781,11 +780,13 @@ int ctlib_timing_collect(ctlib_kernel_stats* out) {
 
 namespace {
 
-// Scratch of the residual step: [Y-line copy | Y-partial image] (the layouts of projection
-// and projection_t, overlapping) followed by the bin tables.
+// Scratch of the residual step: [Y-line copy | partial sinograms] (projection's layout) and
+// [Y-partial image] (projection_t's) overlapping, then the bin tables, then the forward's
+// completion counters (split forward only).
 struct ResidLayout {
     WsLayout la, lb;
-    size_t tables = 0, total = 0;
+    size_t tables = 0, cnt = 0, total = 0;
+    int ncnt = 0;
 };
 
 ResidLayout resid_layout(const ctlib_plan* p, const Tuning& T, int64_t batch, int dtype) {
@@ -793,7 +794,9 @@ ResidLayout resid_layout(const ctlib_plan* p, const Tuning& T, int64_t batch, in
     R.la = layout(p, T, CTLIB_OP_PRJ, batch, dtype);
     R.lb = layout(p, T, CTLIB_OP_PRJ_T, batch, dtype);
     R.tables = R.la.total > R.lb.total ? R.la.total : R.lb.total;
-    R.total = R.tables + align_up((size_t)batch * p->dev.V * (p->dev.D + 2) * sizeof(double2));
+    R.cnt = R.tables + align_up((size_t)batch * p->dev.V * (p->dev.D + 2) * sizeof(double2));
+    if (R.la.fnch > 1) R.ncnt = fwd_resid_items(p->dev, (int)batch, default_fwd_shape(p->dev, false, R.la.fwd_G).G);
+    R.total = R.cnt + align_up((size_t)R.ncnt * sizeof(unsigned int));
     return R;
 }
 
@@ -833,26 +836,24 @@ int ctlib_run_residual_loss(const ctlib_plan* p, int dtype, const void* x, const
     double2* tabs = reinterpret_cast<double2*>(w + R.tables);
     void* yl = w + la.ylines;
     hipError_t e = hipSuccess;
-    if (la.fnch > 1) {
-        // small batch, forward split into line chunks: the partial sinograms are summed into y
-        // (into the first partial when y is not wanted), then the tables of (y - m) * post
-        // (a separate grid-wide sum measured faster than summing inside the per-row table pass)
-        void* yv = y ? y : w + la.sp;
-        e = run_to_sino(p, dt, x, yv, B, false, P.post_prj, la, w, s);
-        if (e == hipSuccess) e = run_tables(p, dt, yv, m, P.post_prj, tabs, B, s, row_sq);
-        if (e == hipSuccess) e = run_to_image(p, dt, tabs, grad, B, false, 1.0, lb, w, s);
-        if (e != hipSuccess) return hip_fail(e, "kernel launch");
-        return CTLIB_OK;
-    }
-    // forward fused with the residual tables, then the adjoint (no separate table pass)
+    // Forward fused with the residual tables, then the adjoint (no separate table pass).  A batch
+    // too small to fill the chip unsplit splits the forward into line chunks: the last chunk of
+    // every work item sums the partial sinograms and forms the tables inside the same launch,
+    // on completion counters cleared by the line-copy launch (or a fill kernel without Y views).
+    unsigned int* cnt = R.ncnt ? reinterpret_cast<unsigned int*>(w + R.cnt) : nullptr;
     if (P.ny) {
         KernelTimer t(K_AUX, 0, 0, s);
-        e = launch_to_ylines(x, yl, dt, B, P.H, P.W, s);
+        e = launch_to_ylines(x, yl, dt, B, P.H, P.W, s, cnt, R.ncnt);
+    } else if (cnt) {
+        KernelTimer t(K_AUX, 0, 0, s);
+        e = launch_zero_u32(cnt, R.ncnt, s);
     }
     if (e == hipSuccess) {
+        FwdShape sh = default_fwd_shape(P, false, la.fwd_G);
+        sh.nch = la.fnch;
         KernelTimer t(K_FWD, sweep_bytes(P, B, dt), 0, s);
-        e = launch_fwd_resid(P, dt, x, P.ny ? yl : nullptr, y, m, tabs, B, default_fwd_shape(P, false, la.fwd_G), s,
-                             row_sq);
+        e = launch_fwd_resid(P, dt, x, P.ny ? yl : nullptr, y, m, tabs, B, sh, s, row_sq,
+                             la.fnch > 1 ? w + la.sp : nullptr, cnt, R.ncnt);
     }
     if (e == hipSuccess) e = run_to_image(p, dt, tabs, grad, B, false, 1.0, lb, w, s);
     if (e != hipSuccess) return hip_fail(e, "kernel launch");

@@ -40,8 +40,12 @@ struct FwdShape {                 // line->bin kernel
 };
 struct AdjShape { int nch; };     // view chunks of the bin->pixel kernel (<= 1: none)
 
-// x [B][H][W] -> xT [B][W][H] with xT[b][c][i] = x[b][H-1-i][c]  (Y-mode lines, bottom to top)
-hipError_t launch_to_ylines(const void* x, void* xT, int dtype, int B, int H, int W, hipStream_t s);
+// x [B][H][W] -> xT [B][W][H] with xT[b][c][i] = x[b][H-1-i][c]  (Y-mode lines, bottom to top);
+// also clears zero[0 .. nzero) (nzero may be 0).
+hipError_t launch_to_ylines(const void* x, void* xT, int dtype, int B, int H, int W, hipStream_t s,
+                            unsigned int* zero = nullptr, int nzero = 0);
+// p[0 .. n) = 0 (a kernel, so that it may be captured in a graph)
+hipError_t launch_zero_u32(unsigned int* p, int n, hipStream_t s);
 
 // Line -> bin operator (projection; weighted = backprojection_t), both sweep axes in one launch.
 // `srcx` holds image rows ([B][H][W]), `srcy` the bottom-to-top column copy ([B][W][H]).
@@ -57,8 +61,14 @@ size_t fwd_lds_bytes(const DevPlan& P, bool weighted, int G);
 // projection fused with the residual bin tables of projection_t: writes y = A x (y may be null)
 // and tabs[b][v] = tables of g = (y - m) * post_prj, so that launch_adj yields A^T (A x - m);
 // rowsq (may be null) receives sum_k (y - m)^2 per row [B][V], y rounded to the data type.
+// shape.nch = k > 1 splits the image lines into k chunks: partial sinograms go to `parts`
+// (k * B*V*D elements) and the last chunk of every work item sums them and forms the tables;
+// `cnt` holds ncnt >= fwd_resid_items() counters that must be zero (they are zero again afterwards).
 hipError_t launch_fwd_resid(const DevPlan& P, int dtype, const void* srcx, const void* srcy, void* y, const void* m,
-                            double2* tabs, int B, FwdShape shape, hipStream_t s, double* rowsq = nullptr);
+                            double2* tabs, int B, FwdShape shape, hipStream_t s, double* rowsq = nullptr,
+                            void* parts = nullptr, unsigned int* cnt = nullptr, int ncnt = 0);
+// Work items (= completion counters) of a launch_fwd_resid with G views per block.
+inline int fwd_resid_items(const DevPlan& P, int B, int G) { return B * ((P.nx + G - 1) / G + (P.ny + G - 1) / G); }
 
 // Weighted line -> bin operator (backprojection_t; post may be null) with wave-private line
 // tables, for lines of at most 1024 pixels (fwd_w_supported); launch_fwd dispatches to it.

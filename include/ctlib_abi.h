@@ -184,8 +184,8 @@ const char* ctlib_batch_last_error(void);
  * cols [n][k] int64 in ascending distance (ties by column), and their weights
  * exp(-dist / median) normalised per point, vals [n][k] float32, where median
  * is the lower median of all n*k selected distances.  Row i of the sparse
- * matrix holds cols[i] / vals[i].  Supported: d <= 128, k <= 64 with
- * d + 2k <= 192 (f32) or 2d + 3k <= 192 (f64).  Asynchronous on `stream`. */
+ * matrix holds cols[i] / vals[i].  Supported: any d, k <= 4096 (k <= n).
+ * Asynchronous on `stream`. */
 size_t ctlib_laplacian_workspace_bytes(int64_t n, int d, int k, int dtype);
 int ctlib_laplacian(const void* x, int64_t n, int d, int k, int dtype, int64_t* cols, float* vals, void* ws,
                     size_t ws_bytes, void* stream);

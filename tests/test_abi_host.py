@@ -160,10 +160,13 @@ def test_laplacian_argument_checks_without_gpu():
     lib.ctlib_laplacian_workspace_bytes.restype = ctypes.c_size_t
     lib.ctlib_laplacian_workspace_bytes.argtypes = [ctypes.c_int64, ctypes.c_int, ctypes.c_int, ctypes.c_int]
     fake = ctypes.c_void_p(0x1000)
-    # k > n, k < 1, unsupported dimension, bad dtype, missing workspace
+    # k > n, k < 1, k above the supported 4096, bad dtype, missing workspace
     assert lib.ctlib_laplacian(fake, 10, 4, 11, 0, fake, fake, fake, 1 << 20, None) == 1
     assert lib.ctlib_laplacian(fake, 10, 4, 0, 0, fake, fake, fake, 1 << 20, None) == 1
-    assert lib.ctlib_laplacian(fake, 10, 300, 4, 0, fake, fake, fake, 1 << 20, None) == 1
+    assert lib.ctlib_laplacian(fake, 10000, 4, 4097, 0, fake, fake, fake, 1 << 20, None) == 1
     assert lib.ctlib_laplacian(fake, 10, 4, 4, 7, fake, fake, fake, 1 << 20, None) == 5
     assert lib.ctlib_laplacian(fake, 10, 4, 4, 0, fake, fake, None, 0, None) == 6
     assert lib.ctlib_laplacian_workspace_bytes(1000, 16, 8, 0) >= 1000 * 4 + 1000 * 8 * 4
+    # any dimension: the general path needs a chunk of distance rows (here all 1000 rows)
+    assert lib.ctlib_laplacian_workspace_bytes(1000, 300, 8, 0) >= 1000 * 1000 * 4
+    assert lib.ctlib_laplacian_workspace_bytes(1000, 16, 4097, 0) == 0

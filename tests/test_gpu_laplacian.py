@@ -55,6 +55,9 @@ def check(x, k, tie_band):
     (1000, 16, 8, torch.float32), (777, 49, 16, torch.float32), (1500, 100, 32, torch.float32),
     (600, 3, 64, torch.float32), (513, 128, 8, torch.float32), (64, 9, 64, torch.float32),
     (900, 16, 8, torch.float64), (700, 49, 16, torch.float64), (300, 30, 32, torch.float64),
+    # general path (dimension or k beyond the register lists): distance rows in chunks, radix select
+    (1200, 256, 128, torch.float32), (700, 300, 16, torch.float32), (900, 20, 200, torch.float32),
+    (500, 129, 65, torch.float64), (3000, 64, 1000, torch.float32), (257, 7, 257, torch.float32),
 ])
 def test_against_oracle(dev, N, d, k, dtype):
     x = torch.rand(N, d, generator=torch.Generator().manual_seed(N + d + k), dtype=dtype).to(dev)
@@ -62,15 +65,18 @@ def test_against_oracle(dev, N, d, k, dtype):
     check(x, k, 1e-5 if dtype == torch.float32 else 1e-12)
 
 
-def test_duplicates_and_determinism(dev):
+@pytest.mark.parametrize("d,k", [(12, 4), (200, 4), (12, 100)])
+def test_duplicates_and_determinism(dev, d, k):
+    """Exact duplicates list the lower column first (register path d=12, k=4; general path for a
+    large dimension or a large k); repeated calls are bitwise identical."""
     g = torch.Generator().manual_seed(4)
-    base = torch.rand(200, 12, generator=g)
+    base = torch.rand(200, d, generator=g)
     x = torch.cat([base, base[:50]]).contiguous().to(dev)   # 50 exact duplicates
-    print("[case] laplacian duplicates", flush=True)
-    W1 = ctlib_amd.laplacian(x, 4)
-    W2 = ctlib_amd.laplacian(x, 4)
+    print(f"[case] laplacian duplicates d={d} k={k}", flush=True)
+    W1 = ctlib_amd.laplacian(x, k)
+    W2 = ctlib_amd.laplacian(x, k)
     assert torch.equal(W1._indices(), W2._indices()) and torch.equal(W1._values(), W2._values())
-    cols = W1._indices()[1].reshape(250, 4).cpu()
+    cols = W1._indices()[1].reshape(250, k).cpu()
     # a duplicated point lists itself and its twin first, lower column first
     for i in range(50):
         assert cols[i, :2].tolist() == [i, 200 + i]
@@ -85,7 +91,7 @@ def test_errors(dev):
     with pytest.raises(ValueError):
         ctlib_amd.laplacian(x, 11)
     with pytest.raises(ValueError):
-        ctlib_amd.laplacian(torch.rand(10, 200, device=dev), 4)      # dimension above the supported 128
+        ctlib_amd.laplacian(torch.rand(5000, 3, device=dev), 4097)    # k above the supported 4096
     with pytest.raises(RuntimeError):
         ctlib_amd.laplacian(torch.rand(10, 4), 4)                     # CPU tensor
     with pytest.raises(TypeError):

@@ -112,3 +112,47 @@ def test_graph_capture_replays(dev):
         torch.cuda.synchronize()
         assert torch.equal(out, ref), rep
         del junk
+
+
+def test_captured_plan_survives_cache_clear_and_eviction(dev):
+    """A graph captured through the public API keeps its plan's device tables alive: after
+    clear_plan_cache() and after cycling more geometries than the cache holds (least recently
+    used evicted), replays still equal the eager result."""
+    from ctlib_amd import _C
+
+    o = torch.from_numpy(phantom.fan_options(120, 97, 64, 64, 0.02, 0.03, 5.95, 4.906, 0))
+    x = torch.from_numpy(phantom.phantom_batch(2, 64, 64, config=1, scale=0.02)).to(dev)
+    ref = ctlib_amd.projection_t(ctlib_amd.projection(x, o), o)
+    print("[case] captured plan survives clear / eviction", flush=True)
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        ctlib_amd.projection_t(ctlib_amd.projection(x, o), o)
+    torch.cuda.current_stream().wait_stream(s)
+    torch.cuda.synchronize()
+    pinned0 = _C.plan_cache_info()["pinned"]
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        out = ctlib_amd.projection_t(ctlib_amd.projection(x, o), o)
+    info = _C.plan_cache_info()
+    assert info["pinned"] == pinned0 + 1  # one geometry, pinned once
+    _C.clear_plan_cache()
+    assert _C.plan_cache_info()["entries"] == 0
+    xs = torch.rand(1, 1, 16, 16, device=dev)
+    cap = info["capacity"]
+    for k in range(cap + 6):  # more geometries than the cache holds
+        ok = torch.from_numpy(phantom.fan_options(8 + k, 11, 16, 16, 0.1, 0.13, 3.0, 2.0, 0))
+        ctlib_amd.projection(xs, ok)
+    info2 = _C.plan_cache_info()
+    assert info2["entries"] == cap
+    for rep in range(3):
+        out.zero_()
+        graph.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(out, ref), rep
+    # least recently used first: the newest geometries are still cached (no rebuild)
+    created = _C.plan_cache_info()["created"]
+    for k in range(cap + 6 - 8, cap + 6):
+        ok = torch.from_numpy(phantom.fan_options(8 + k, 11, 16, 16, 0.1, 0.13, 3.0, 2.0, 0))
+        ctlib_amd.projection(xs, ok)
+    assert _C.plan_cache_info()["created"] == created
