@@ -563,8 +563,9 @@ def main():
                 float(torch.linalg.vector_norm((g_ag - g_plain).double()) / ref))
 
     stats = None
+    plain_stats = None
     if "plain" in paths:
-        results["plain"], _ = timed(step_plain, args.steps, args.warmup)
+        results["plain"], plain_stats = timed(step_plain, args.steps, args.warmup, collect=True)
     results["autograd"], stats = timed(step_autograd, args.steps, args.warmup, collect=True)
     if "graph" in paths:
         step_fused()
@@ -620,9 +621,13 @@ def main():
         "sweep_equiv_frac_8TBs": round(achieved / HBM_PEAK_GBS, 4),
         # compulsory bytes of one operator application: the image and the sinogram once each
         "compulsory_bytes_per_launch": 4 * B * (512 * 512 + 736 * 736)})
-    kernels = {c: {"launches": stats[c]["launches"],
-                   "ms_per_launch": round(stats[c]["ms"] / max(stats[c]["launches"], 1), 4)}
-               for c in ("fwd", "adj", "ramp", "aux")}
+    def per_step(st):
+        # library kernels per step of a path: launches and milliseconds by class (HIP events)
+        return {c: {"launches_per_step": round(st[c]["launches"] / args.steps, 2),
+                    "ms_per_step": round(st[c]["ms"] / args.steps, 4)} for c in ("fwd", "adj", "aux")}
+    kernels = {"autograd": per_step(stats)}
+    if plain_stats:
+        kernels["plain"] = per_step(plain_stats)
     step_bytes = stats["fwd"]["algo_bytes"] / max(stats["fwd"]["launches"], 1) + \
         stats["adj"]["algo_bytes"] / max(stats["adj"]["launches"], 1)
 

@@ -781,12 +781,10 @@ int ctlib_timing_collect(ctlib_kernel_stats* out) {
 namespace {
 
 // Scratch of the residual step: [Y-line copy | partial sinograms] (projection's layout) and
-// [Y-partial image] (projection_t's) overlapping, then the bin tables, then the forward's
-// completion counters (split forward only).
+// [Y-partial image] (projection_t's) overlapping, then the bin tables.
 struct ResidLayout {
     WsLayout la, lb;
-    size_t tables = 0, cnt = 0, total = 0;
-    int ncnt = 0;
+    size_t tables = 0, total = 0;
 };
 
 ResidLayout resid_layout(const ctlib_plan* p, const Tuning& T, int64_t batch, int dtype) {
@@ -794,9 +792,7 @@ ResidLayout resid_layout(const ctlib_plan* p, const Tuning& T, int64_t batch, in
     R.la = layout(p, T, CTLIB_OP_PRJ, batch, dtype);
     R.lb = layout(p, T, CTLIB_OP_PRJ_T, batch, dtype);
     R.tables = R.la.total > R.lb.total ? R.la.total : R.lb.total;
-    R.cnt = R.tables + align_up((size_t)batch * p->dev.V * (p->dev.D + 2) * sizeof(double2));
-    if (R.la.fnch > 1) R.ncnt = fwd_resid_items(p->dev, (int)batch, default_fwd_shape(p->dev, false, R.la.fwd_G).G);
-    R.total = R.cnt + align_up((size_t)R.ncnt * sizeof(unsigned int));
+    R.total = R.tables + align_up((size_t)batch * p->dev.V * (p->dev.D + 2) * sizeof(double2));
     return R;
 }
 
@@ -836,24 +832,31 @@ int ctlib_run_residual_loss(const ctlib_plan* p, int dtype, const void* x, const
     double2* tabs = reinterpret_cast<double2*>(w + R.tables);
     void* yl = w + la.ylines;
     hipError_t e = hipSuccess;
-    // Forward fused with the residual tables, then the adjoint (no separate table pass).  A batch
-    // too small to fill the chip unsplit splits the forward into line chunks: the last chunk of
-    // every work item sums the partial sinograms and forms the tables inside the same launch,
-    // on completion counters cleared by the line-copy launch (or a fill kernel without Y views).
-    unsigned int* cnt = R.ncnt ? reinterpret_cast<unsigned int*>(w + R.cnt) : nullptr;
     if (P.ny) {
         KernelTimer t(K_AUX, 0, 0, s);
-        e = launch_to_ylines(x, yl, dt, B, P.H, P.W, s, cnt, R.ncnt);
-    } else if (cnt) {
-        KernelTimer t(K_AUX, 0, 0, s);
-        e = launch_zero_u32(cnt, R.ncnt, s);
+        e = launch_to_ylines(x, yl, dt, B, P.H, P.W, s);
     }
-    if (e == hipSuccess) {
+    if (e == hipSuccess && la.fnch > 1) {
+        // Batch too small to fill the chip unsplit: the forward is split into line chunks, and
+        // one pass sums the partial sinograms (FP64, chunk order), writes y if wanted and forms the
+        // residual tables and row sums.  (Forming them instead in the last chunk of every work
+        // item, on completion counters, measured 0.17 ms slower at fan-ED 512^2 B=32.)
         FwdShape sh = default_fwd_shape(P, false, la.fwd_G);
         sh.nch = la.fnch;
+        {
+            KernelTimer t(K_FWD, sweep_bytes(P, B, dt), 0, s);
+            e = launch_fwd(P, dt, x, P.ny ? yl : nullptr, w + la.sp, B, false, P.post_prj, sh, s);
+        }
+        if (e == hipSuccess) {
+            KernelTimer t(K_AUX, 0, 0, s);
+            e = launch_bin_tables(P, dt, w + la.sp, m, P.post_prj, 1.0, tabs, B, s, row_sq, la.fnch,
+                                  (size_t)B * P.V * P.D, y);
+        }
+    } else if (e == hipSuccess) {
+        // forward fused with the residual tables (no separate table pass)
         KernelTimer t(K_FWD, sweep_bytes(P, B, dt), 0, s);
-        e = launch_fwd_resid(P, dt, x, P.ny ? yl : nullptr, y, m, tabs, B, sh, s, row_sq,
-                             la.fnch > 1 ? w + la.sp : nullptr, cnt, R.ncnt);
+        e = launch_fwd_resid(P, dt, x, P.ny ? yl : nullptr, y, m, tabs, B, default_fwd_shape(P, false, la.fwd_G), s,
+                             row_sq);
     }
     if (e == hipSuccess) e = run_to_image(p, dt, tabs, grad, B, false, 1.0, lb, w, s);
     if (e != hipSuccess) return hip_fail(e, "kernel launch");

@@ -283,73 +283,6 @@ __device__ __forceinline__ void fwd_store(const DevPlan& P, const double (&S)[VP
     }
 }
 
-// Last line chunk of a split residual forward work item (views vw of this wave): every chunk
-// has stored its partial bin values (post applied, rounded to T) at parts + c * pstride in
-// output order; here y_k = sum over the chunks in chunk order (FP64, so the result does not
-// depend on which chunk finished last), y is written if wanted, and the residual bin tables
-// of projection_t and the row sums of (y - m)^2 are formed as fwd_store does with RESID.
-// Bins go 64 at a time (coalesced).  All loads of a view are issued before its one serial
-// part, the wave scan over the NCK bin groups (NCK * 64 >= D), so the block pays one memory
-// latency per view instead of one per bin group.
-template <typename T, int VPW, int NCK>
-__device__ __forceinline__ void fwd_finalize_resid(const DevPlan& P, const int (&vw)[VPW], int grp, int b, int mode,
-                                                   int lane, int wave, const T* __restrict__ parts, size_t pstride,
-                                                   int nch, T* __restrict__ y, const T* __restrict__ m,
-                                                   double2* __restrict__ tabs, double* __restrict__ rowsq) {
-    constexpr int G = 4 * VPW;
-    const int nv = mode == MODE_Y ? P.ny : P.nx;
-    const int D = P.D;
-#pragma unroll
-    for (int v = 0; v < VPW; v++) {
-        const int idx = grp * G + wave * VPW + v;
-        if (idx >= nv) continue;  // wave-uniform
-        const int view = vw[v];
-        const bool rev = P.vp[view].rev;
-        const size_t row = (size_t)b * P.V + view;
-        const double* pr = P.post_prj + (size_t)view * D;
-        const double* ev = P.edges + (size_t)view * (D + 1);
-        double2* tb = tabs + row * (D + 2);
-        const T* pp = parts + row * D;
-        // per bin: g = (y - m) * post, its width-weighted value and e_k * g
-        double gk[NCK], wg[NCK], eg[NCK];
-        double rsq = 0.0;
-#pragma unroll
-        for (int i = 0; i < NCK; i++) {
-            const int k = lane + 64 * i;
-            gk[i] = wg[i] = eg[i] = 0.0;
-            if (k < D) {
-                const int o = rev ? D - 1 - k : k;
-                double yk = (double)pp[o];
-                for (int c = 1; c < nch; c++) yk += (double)pp[c * pstride + o];
-                if (y) y[row * D + o] = (T)yk;
-                const double mk = (double)m[row * D + o];
-                const double ek = ev[k];
-                gk[i] = (yk - mk) * pr[k];
-                const double rk = (double)(T)yk - mk;
-                rsq = __builtin_fma(rk, rk, rsq);
-                wg[i] = (ev[k + 1] - ek) * gk[i];
-                eg[i] = ek * gk[i];
-            }
-        }
-        double carry = 0.0;
-#pragma unroll
-        for (int i = 0; i < NCK; i++) {
-            const int k = lane + 64 * i;
-            const double inc = wave_scan_incl(wg[i], lane);
-            if (k < D) tb[k + 1] = make_double2(carry + (inc - wg[i]) - eg[i], gk[i]);
-            carry += readlane_d(inc, 63);
-        }
-        if (lane == 0) {
-            tb[0] = make_double2(0.0, 0.0);
-            tb[D + 1] = make_double2(carry, 0.0);
-        }
-        if (rowsq) {
-            const double tot = readlane_d(wave_scan_incl(rsq, lane), 63);
-            if (lane == 0) rowsq[row] = tot;
-        }
-    }
-}
-
 // Builds NTAB line tables {C_j, slope_j}, j = 0..n+1, table t from val[t] (this
 // thread's pixels p0 .. p0+m-1; already rounded to the storage precision of
 // the slope).  Table t starts at tb + t*tabn; `ws` holds NTAB x 4 wave sums.

@@ -17,14 +17,10 @@ using namespace dd;
 
 namespace {
 
-// zero[0 .. nzero) is cleared by the first block (completion counters of the launch that follows).
 template <typename T>
-__global__ __launch_bounds__(256) void k_to_ylines(const T* __restrict__ x, T* __restrict__ xT, int H, int W,
-                                                   unsigned int* __restrict__ zero, int nzero) {
+__global__ __launch_bounds__(256) void k_to_ylines(const T* __restrict__ x, T* __restrict__ xT, int H, int W) {
     __shared__ T tile[32][33];
     const int b = blockIdx.z;
-    if (nzero > 0 && blockIdx.x == 0 && blockIdx.y == 0 && b == 0)
-        for (int i = threadIdx.x; i < nzero; i += 256) zero[i] = 0u;
     ylines_tile<T>(x + (size_t)b * H * W, xT + (size_t)b * H * W, H, W, blockIdx.y * 32, blockIdx.x * 32, tile);
 }
 
@@ -37,68 +33,22 @@ __global__ __launch_bounds__(256) void k_to_ylines(const T* __restrict__ x, T* _
 template <typename T, int NEM, int M, bool WEIGHTED>
 constexpr int fwd_waves() { return sizeof(T) == 4 && !WEIGHTED && NEM > 20 && NEM <= 24 && M <= 4 ? 3 : 1; }
 
-// Line -> bin launch.  RESID with nch > 1 (residual step of a batch too small to fill the chip unsplit): each chunk
-// stores its partial sinogram at y + ch*B*V*D, then counts itself done on cnt[work item]
-// (agent scope, so the partials are visible on every XCD); the last chunk of the work item sums
-// the partials in chunk order and forms the residual tables (fwd_finalize_resid), writes yout if
-// it is not null, and resets the count for the next launch.  No block waits for another.
 template <typename T, int VPW, int NEM, int M, bool WEIGHTED, bool RESID, bool FULL, bool EDGEX>
 __global__ __launch_bounds__(256, (fwd_waves<T, NEM, M, WEIGHTED>())) void k_fwd(DevPlan P, const T* __restrict__ srcx, const T* __restrict__ srcy,
                                              T* __restrict__ y, int B, int ngx, int ngy,
                                              const double* __restrict__ post, const T* __restrict__ m,
-                                             double2* __restrict__ tabs, int nch, double* __restrict__ rowsq,
-                                             T* __restrict__ yout, unsigned int* __restrict__ cnt) {
+                                             double2* __restrict__ tabs, int nch, double* __restrict__ rowsq) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int nwx = B * ngx;
     int w = xcd_remap(blockIdx.x, gridDim.x, (nwx + B * ngy) * nch);
     if (w < 0) return;
     const int ch = w % nch;
     w /= nch;
-    const int item = w;
     const int mode = w < nwx ? MODE_X : MODE_Y;
     if (mode == MODE_Y) w -= nwx;
     const int ngroups = mode == MODE_Y ? ngy : ngx;
     const int b = w / ngroups, grp = w - b * ngroups;
     const int nl = mode == MODE_Y ? P.W : P.H;
-    if constexpr (RESID) {
-        if (nch > 1) {
-            const size_t pstride = (size_t)B * P.V * P.D;
-            fwd_block<T, VPW, NEM, M, WEIGHTED, false, FULL, EDGEX>(P, mode == MODE_Y ? srcy : srcx, y + ch * pstride, b,
-                                                                    mode, grp, post, nullptr, nullptr, smem,
-                                                                    ch * nl / nch, (ch + 1) * nl / nch, nullptr);
-            __shared__ int s_last;
-            // the barrier completes every wave's partial stores before thread 0's release
-            __syncthreads();
-            if (threadIdx.x == 0) {
-                // release by every chunk; only the last one, which reads the others' partials,
-                // acquires (a fence after its read of the count: the earlier releases
-                // synchronise with it)
-                const unsigned int old =
-                    __hip_atomic_fetch_add(cnt + item, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-                const bool last = old == (unsigned int)(nch - 1);
-                if (last) {
-                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-                    __hip_atomic_store(cnt + item, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                }
-                s_last = last;
-            }
-            __syncthreads();
-            if (!s_last) return;
-            constexpr int G = 4 * VPW;
-            const int lane = threadIdx.x & 63;
-            const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-            const int* vlist = mode == MODE_Y ? P.yviews : P.xviews;
-            const int nv = mode == MODE_Y ? P.ny : P.nx;
-            int vw[VPW];
-#pragma unroll
-            for (int v = 0; v < VPW; v++) {
-                const int idx = grp * G + wave * VPW + v;
-                vw[v] = vlist[idx < nv ? idx : nv - 1];
-            }
-            fwd_finalize_resid<T, VPW, NEM>(P, vw, grp, b, mode, lane, wave, y, pstride, nch, yout, m, tabs, rowsq);
-            return;
-        }
-    }
     fwd_block<T, VPW, NEM, M, WEIGHTED, RESID, FULL, EDGEX>(P, mode == MODE_Y ? srcy : srcx,
                                                             y + (size_t)ch * B * P.V * P.D, b, mode, grp, post, m,
                                                             tabs, smem, ch * nl / nch, (ch + 1) * nl / nch, rowsq);
@@ -116,12 +66,15 @@ __global__ __launch_bounds__(256) void k_sum_parts(T* out, const T* parts, int n
 }
 
 // Per-(b, view) bin tables {C_j, g_j} of g_k = ((y - m)[o(k)] * pre[k]) * scale, j = 0..D+1;
-// with m and rowsq not null also rowsq[row] = sum_k (y - m)_k^2.
+// with m and rowsq not null also rowsq[row] = sum_k (y - m)_k^2.  nparts > 1: y is the sum of
+// nparts partial sinograms pstride apart (FP64, in order), also stored to yout if not null;
+// rowsq then uses the sum rounded to TIn, as a stored y would be.
 template <typename TIn>
 __global__ __launch_bounds__(256) void k_bin_tables(DevPlan P, const TIn* __restrict__ y, const TIn* __restrict__ m,
                                                     const double* __restrict__ pre, double scale,
                                                     double2* __restrict__ tabs, int nrows,
-                                                    double* __restrict__ rowsq) {
+                                                    double* __restrict__ rowsq, int nparts, size_t pstride,
+                                                    TIn* __restrict__ yout) {
     const int lane = threadIdx.x & 63;
     const int row = blockIdx.x * 4 + (threadIdx.x >> 6);  // b*V + v
     if (row >= nrows) return;                              // whole wave; no barriers below
@@ -130,10 +83,12 @@ __global__ __launch_bounds__(256) void k_bin_tables(DevPlan P, const TIn* __rest
     const bool rev = P.vp[v].rev;
     const double* e = P.edges + (size_t)v * (D + 1);
     const TIn* yr = y + (size_t)row * D;
+    TIn* yo = yout ? yout + (size_t)row * D : nullptr;
     const TIn* mr = m ? m + (size_t)row * D : nullptr;
     const double* pr = pre ? pre + (size_t)v * D : nullptr;
     double2* tb = tabs + (size_t)row * (D + 2);
     // 64 consecutive bins per step: coalesced loads and stores, one wave scan per step
+    // (issuing four steps' loads before their scans measured slower)
     double carry = 0.0, rsq = 0.0;
     for (int k0 = 0; k0 < D; k0 += 64) {
         const int k = k0 + lane;
@@ -141,8 +96,17 @@ __global__ __launch_bounds__(256) void k_bin_tables(DevPlan P, const TIn* __rest
         if (k < D) {
             const int o = rev ? D - 1 - k : k;
             g = (double)yr[o];
-            if (mr) g -= (double)mr[o];
-            rsq = __builtin_fma(g, g, rsq);
+            double r = 0.0;
+            if (nparts > 1) {
+                for (int c = 1; c < nparts; c++) g += (double)yr[c * pstride + o];
+                if (yo) yo[o] = (TIn)g;
+                if (mr) r = (double)(TIn)g - (double)mr[o];
+            }
+            if (mr) {
+                g -= (double)mr[o];
+                if (nparts <= 1) r = g;
+            }
+            rsq = __builtin_fma(r, r, rsq);
             if (pr) g *= pr[k];
             g *= scale;
             ek = e[k];
@@ -177,27 +141,13 @@ __global__ __launch_bounds__(256) void k_combine(T* __restrict__ out, const T* _
 // Host launchers
 // ---------------------------------------------------------------------------
 
-hipError_t launch_to_ylines(const void* x, void* xT, int dtype, int B, int H, int W, hipStream_t s,
-                            unsigned int* zero, int nzero) {
+hipError_t launch_to_ylines(const void* x, void* xT, int dtype, int B, int H, int W, hipStream_t s) {
     if (B <= 0) return hipSuccess;
     dim3 grid((W + 31) / 32, (H + 31) / 32, B);
     if (dtype == DT_F32)
-        hipLaunchKernelGGL(k_to_ylines<float>, grid, dim3(256), 0, s, (const float*)x, (float*)xT, H, W, zero, nzero);
+        hipLaunchKernelGGL(k_to_ylines<float>, grid, dim3(256), 0, s, (const float*)x, (float*)xT, H, W);
     else
-        hipLaunchKernelGGL(k_to_ylines<double>, grid, dim3(256), 0, s, (const double*)x, (double*)xT, H, W, zero,
-                           nzero);
-    return hipGetLastError();
-}
-
-namespace {
-__global__ __launch_bounds__(256) void k_zero_u32(unsigned int* __restrict__ p, int n) {
-    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) p[i] = 0u;
-}
-}  // namespace
-
-hipError_t launch_zero_u32(unsigned int* p, int n, hipStream_t s) {
-    if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_zero_u32, dim3((n + 255) / 256 < 64 ? (n + 255) / 256 : 64), dim3(256), 0, s, p, n);
+        hipLaunchKernelGGL(k_to_ylines<double>, grid, dim3(256), 0, s, (const double*)x, (double*)xT, H, W);
     return hipGetLastError();
 }
 
@@ -219,9 +169,6 @@ struct FwdArgs {
     double2* tabs;
     int nch;
     double* rowsq;
-    void* yout;          // split residual forward: the summed y (may be null); y holds the partials
-    unsigned int* cnt;   // split residual forward: completion counts, one per work item, zero
-    int ncnt;            // counters available at cnt
 };
 
 template <typename T, int VPW, int NEM, int M, bool WTD, bool RESID, bool FULL, bool EDGEX>
@@ -231,7 +178,6 @@ hipError_t fwd_launch_t(const DevPlan& P, const FwdArgs& a, hipStream_t s) {
     const long long nw = (long long)(ngx + ngy) * a.B * a.nch;
     if (nw > (1LL << 30)) return hipErrorInvalidValue;  // grid size (one block per work item)
     const int nwork = (int)nw;
-    if (a.cnt && (long long)(ngx + ngy) * a.B > a.ncnt) return hipErrorInvalidValue;  // one counter per work item
     const size_t lds = fwd_lds_bytes(P, WTD, G);
     auto kern = k_fwd<T, VPW, NEM, M, WTD, RESID, FULL, EDGEX>;
     if (lds > 64 * 1024) {
@@ -239,7 +185,7 @@ hipError_t fwd_launch_t(const DevPlan& P, const FwdArgs& a, hipStream_t s) {
         if (e != hipSuccess) return e;
     }
     hipLaunchKernelGGL(kern, dim3(grid_for(nwork)), dim3(NT), lds, s, P, (const T*)a.srcx, (const T*)a.srcy, (T*)a.y,
-                       a.B, ngx, ngy, a.post, (const T*)a.m, a.tabs, a.nch, a.rowsq, (T*)a.yout, a.cnt);
+                       a.B, ngx, ngy, a.post, (const T*)a.m, a.tabs, a.nch, a.rowsq);
     return hipGetLastError();
 }
 
@@ -299,21 +245,16 @@ hipError_t launch_fwd(const DevPlan& P, int dtype, const void* srcx, const void*
     const int nch = shape.nch < 1 ? 1 : shape.nch;
     if ((P.nx && nch > P.H) || (P.ny && nch > P.W)) return hipErrorInvalidValue;
     if (weighted && fwd_w_supported(P)) return launch_fwd_w(P, dtype, srcx, srcy, y, B, post, nch, s);
-    FwdArgs a{srcx, srcy, y, B, post, nullptr, nullptr, nch, nullptr, nullptr, nullptr, 0};
+    FwdArgs a{srcx, srcy, y, B, post, nullptr, nullptr, nch, nullptr};
     if (dtype == DT_F32) return fwd_dispatch<float>(P, a, weighted, false, shape.G, s);
     return fwd_dispatch<double>(P, a, weighted, false, shape.G, s);
 }
 
 hipError_t launch_fwd_resid(const DevPlan& P, int dtype, const void* srcx, const void* srcy, void* y, const void* m,
-                            double2* tabs, int B, FwdShape shape, hipStream_t s, double* rowsq, void* parts,
-                            unsigned int* cnt, int ncnt) {
+                            double2* tabs, int B, FwdShape shape, hipStream_t s, double* rowsq) {
     if (B <= 0 || P.nx + P.ny == 0) return hipSuccess;
     if ((P.nx && P.W > NT * 8) || (P.ny && P.H > NT * 8) || P.D > 2046) return hipErrorInvalidValue;
-    const int nch = shape.nch < 1 ? 1 : shape.nch;
-    if ((P.nx && nch > P.H) || (P.ny && nch > P.W)) return hipErrorInvalidValue;
-    if (nch > 1 && (!parts || !cnt)) return hipErrorInvalidValue;
-    FwdArgs a = nch > 1 ? FwdArgs{srcx, srcy, parts, B, P.post_prj, m, tabs, nch, rowsq, y, cnt, ncnt}
-                        : FwdArgs{srcx, srcy, y, B, P.post_prj, m, tabs, 1, rowsq, nullptr, nullptr, 0};
+    FwdArgs a{srcx, srcy, y, B, P.post_prj, m, tabs, 1, rowsq};
     if (dtype == DT_F32) return fwd_dispatch<float>(P, a, false, true, shape.G, s);
     return fwd_dispatch<double>(P, a, false, true, shape.G, s);
 }
@@ -332,16 +273,17 @@ hipError_t launch_sum_parts(void* out, const void* parts, int dtype, int nch, si
 }
 
 hipError_t launch_bin_tables(const DevPlan& P, int in_dtype, const void* y, const void* m, const double* pre,
-                             double scale, double2* tabs, int B, hipStream_t s, double* rowsq) {
+                             double scale, double2* tabs, int B, hipStream_t s, double* rowsq, int nparts,
+                             size_t pstride, void* yout) {
     const int nrows = B * P.V;
     if (nrows <= 0) return hipSuccess;
     dim3 grid((nrows + 3) / 4);
     if (in_dtype == DT_F32)
         hipLaunchKernelGGL(k_bin_tables<float>, grid, dim3(256), 0, s, P, (const float*)y, (const float*)m, pre, scale,
-                           tabs, nrows, rowsq);
+                           tabs, nrows, rowsq, nparts, pstride, (float*)yout);
     else
         hipLaunchKernelGGL(k_bin_tables<double>, grid, dim3(256), 0, s, P, (const double*)y, (const double*)m, pre,
-                           scale, tabs, nrows, rowsq);
+                           scale, tabs, nrows, rowsq, nparts, pstride, (double*)yout);
     return hipGetLastError();
 }
 

@@ -40,12 +40,8 @@ struct FwdShape {                 // line->bin kernel
 };
 struct AdjShape { int nch; };     // view chunks of the bin->pixel kernel (<= 1: none)
 
-// x [B][H][W] -> xT [B][W][H] with xT[b][c][i] = x[b][H-1-i][c]  (Y-mode lines, bottom to top);
-// also clears zero[0 .. nzero) (nzero may be 0).
-hipError_t launch_to_ylines(const void* x, void* xT, int dtype, int B, int H, int W, hipStream_t s,
-                            unsigned int* zero = nullptr, int nzero = 0);
-// p[0 .. n) = 0 (a kernel, so that it may be captured in a graph)
-hipError_t launch_zero_u32(unsigned int* p, int n, hipStream_t s);
+// x [B][H][W] -> xT [B][W][H] with xT[b][c][i] = x[b][H-1-i][c]  (Y-mode lines, bottom to top)
+hipError_t launch_to_ylines(const void* x, void* xT, int dtype, int B, int H, int W, hipStream_t s);
 
 // Line -> bin operator (projection; weighted = backprojection_t), both sweep axes in one launch.
 // `srcx` holds image rows ([B][H][W]), `srcy` the bottom-to-top column copy ([B][W][H]).
@@ -61,14 +57,8 @@ size_t fwd_lds_bytes(const DevPlan& P, bool weighted, int G);
 // projection fused with the residual bin tables of projection_t: writes y = A x (y may be null)
 // and tabs[b][v] = tables of g = (y - m) * post_prj, so that launch_adj yields A^T (A x - m);
 // rowsq (may be null) receives sum_k (y - m)^2 per row [B][V], y rounded to the data type.
-// shape.nch = k > 1 splits the image lines into k chunks: partial sinograms go to `parts`
-// (k * B*V*D elements) and the last chunk of every work item sums them and forms the tables;
-// `cnt` holds ncnt >= fwd_resid_items() counters that must be zero (they are zero again afterwards).
 hipError_t launch_fwd_resid(const DevPlan& P, int dtype, const void* srcx, const void* srcy, void* y, const void* m,
-                            double2* tabs, int B, FwdShape shape, hipStream_t s, double* rowsq = nullptr,
-                            void* parts = nullptr, unsigned int* cnt = nullptr, int ncnt = 0);
-// Work items (= completion counters) of a launch_fwd_resid with G views per block.
-inline int fwd_resid_items(const DevPlan& P, int B, int G) { return B * ((P.nx + G - 1) / G + (P.ny + G - 1) / G); }
+                            double2* tabs, int B, FwdShape shape, hipStream_t s, double* rowsq = nullptr);
 
 // Weighted line -> bin operator (backprojection_t; post may be null) with wave-private line
 // tables, for lines of at most 1024 pixels (fwd_w_supported); launch_fwd dispatches to it.
@@ -77,9 +67,12 @@ hipError_t launch_fwd_w(const DevPlan& P, int dtype, const void* srcx, const voi
                         const double* post, int nch, hipStream_t s);
 
 // Per (b, view) cumulative-integral tables {C_j, g_j}, j = 0..D+1, of g_k = (y - m)[o(k)] * pre[k] * scale;
-// with m and rowsq not null also rowsq[b][v] = sum_k (y - m)_k^2.
+// with m and rowsq not null also rowsq[b][v] = sum_k (y - m)_k^2 (y rounded to the data type).
+// nparts > 1: y holds nparts partial sinograms pstride elements apart, summed in FP64 in order,
+// and the sum is written to yout (rounded to the data type) when yout is not null.
 hipError_t launch_bin_tables(const DevPlan& P, int in_dtype, const void* y, const void* m, const double* pre,
-                             double scale, double2* tabs, int B, hipStream_t s, double* rowsq = nullptr);
+                             double scale, double2* tabs, int B, hipStream_t s, double* rowsq = nullptr,
+                             int nparts = 1, size_t pstride = 0, void* yout = nullptr);
 
 // Bin -> pixel operator (projection_t; weighted = backprojection, pixel weight * wscale), both axes in
 // one launch.  X lines are written to outX [B][H][W]; Y lines to outYT [B][W][H] (bottom-to-top per column).

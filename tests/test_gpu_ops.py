@@ -156,3 +156,24 @@ def test_captured_plan_survives_cache_clear_and_eviction(dev):
         ok = torch.from_numpy(phantom.fan_options(8 + k, 11, 16, 16, 0.1, 0.13, 3.0, 2.0, 0))
         ctlib_amd.projection(xs, ok)
     assert _C.plan_cache_info()["created"] == created
+
+
+def test_reference_module_name_runs_unchanged(dev):
+    """A script written against the reference (`import ctlib`, device options of the data dtype as
+    ctlib.cpp:17 requires) runs unchanged and gives exactly the library's results."""
+    import ctlib
+
+    o = torch.from_numpy(phantom.fan_options(90, 80, 64, 64, 0.02, 0.03, 5.95, 4.906, 0)).float().to(dev)
+    x = torch.from_numpy(phantom.phantom_batch(2, 64, 64, config=1, scale=0.02)).to(dev)
+    print("[case] import ctlib", flush=True)
+    sino = ctlib.projection(x, o)
+    outs = {"projection": sino, "projection_t": ctlib.projection_t(sino, o),
+            "backprojection": ctlib.backprojection(sino, o), "backprojection_t": ctlib.backprojection_t(x, o),
+            "fbp": ctlib.fbp(sino, o), "backprojection_sv": ctlib.backprojection_sv(sino, o)}
+    for name, got in outs.items():
+        inp = x if name in ("projection", "backprojection_t") else sino
+        assert torch.equal(got, getattr(ctlib_amd, name)(inp, o)), name
+    pts = torch.rand(300, 5, device=dev)
+    W = ctlib.laplacian(pts, 6)
+    W2 = ctlib_amd.laplacian(pts, 6)
+    assert torch.equal(W._indices(), W2._indices()) and torch.equal(W._values(), W2._values())
