@@ -534,6 +534,7 @@ int ctlib_plan_create(const ctlib_geom* geom, int device, ctlib_plan** out) {
     const size_t o_taps = take(sizeof(double) * H.ramp_taps.size());
     const size_t o_pw = take(sizeof(double) * H.ramp_preweight.size());
     const size_t o_lkd = take(sizeof(double) * H.lookup_d.size());
+    const size_t o_alk = take(sizeof(float) * H.adj_lk.size());
     const size_t total = off;
 
     DeviceGuard guard(device);
@@ -559,6 +560,7 @@ int ctlib_plan_create(const ctlib_geom* geom, int device, ctlib_plan** out) {
         {o_taps, H.ramp_taps.data(), sizeof(double) * H.ramp_taps.size()},
         {o_pw, H.ramp_preweight.data(), sizeof(double) * H.ramp_preweight.size()},
         {o_lkd, H.lookup_d.data(), sizeof(double) * H.lookup_d.size()},
+        {o_alk, H.adj_lk.data(), sizeof(float) * H.adj_lk.size()},
     };
     for (const Copy& c : copies) {
         if (!c.bytes) continue;
@@ -581,6 +583,7 @@ int ctlib_plan_create(const ctlib_geom* geom, int device, ctlib_plan** out) {
     P.taps = reinterpret_cast<const double*>(base + o_taps);
     P.preweight = reinterpret_cast<const double*>(base + o_pw);
     P.lookup_d = reinterpret_cast<const double*>(base + o_lkd);
+    P.adj_lk = reinterpret_cast<const float4*>(base + o_alk);
     P.nx = (int)H.xviews.size();
     P.ny = (int)H.yviews.size();
     P.V = V;
@@ -675,6 +678,7 @@ int ctlib_plan_export(const ctlib_geom* geom, int table, void* dst, size_t* byte
     case 6: src = H.yviews.data(); n = H.yviews.size() * sizeof(int); break;
     case 7: src = H.ramp_taps.data(); n = H.ramp_taps.size() * sizeof(double); break;
     case 8: src = H.ramp_preweight.data(); n = H.ramp_preweight.size() * sizeof(double); break;
+    case 10: src = H.adj_lk.data(); n = H.adj_lk.size() * sizeof(float); break;
     case 9:
         ea[0] = H.ea_c[0]; ea[1] = H.ea_c[1]; ea[2] = H.ea_c[2]; ea[3] = H.ea_c[3]; ea[4] = H.ea_c[4];
         ea[5] = H.ea_c[5]; ea[6] = H.ea_T; ea[7] = (float)H.ea_poly;

@@ -664,9 +664,59 @@ __device__ __forceinline__ double eval_G(const double2* __restrict__ t, double a
     return __builtin_fma(pi, c.y, c.x);
 }
 
+// The lookup of one view composed with one line's pixel-edge map, fp32 data.  The
+// plan stores {64 c0, c1, 64 c2, c3} (HostPlan::adj_lk holds c0..c3 scaled so):
+// at pixel edge p = lane + 64*i the lookup argument is (n0 + i C0) / (d0 + i C2)
+// with n0 = C0 lane/64 + c1, d0 = C2 lane/64 + c3 (one fp32 FMA each with the
+// literal i), and the axis coordinate pi0 + i (64 s) is stepped by one fp64 add
+// per slot.  No edge position is converted to fp32 and no per-slot pixel index
+// is held in registers.
+struct LineLk {
+    double pi0, s64;
+    float n0, C0, d0, C2;
+};
+__device__ __forceinline__ LineLk line_lk(double a, double s, float4 c, int lane) {
+    LineLk k;
+    k.pi0 = __builtin_fma(s, (double)lane, a);
+    k.s64 = 64.0 * s;
+    const float lq = (float)lane * (1.0f / 64.0f);  // exact
+    k.n0 = __builtin_fmaf(c.x, lq, c.y);
+    k.C0 = c.x;
+    k.d0 = __builtin_fmaf(c.z, lq, c.w);
+    k.C2 = c.z;
+    return k;
+}
+
+// eval_G of pixel edge lane + 64*i of a line at axis coordinate pi through its
+// LineLk (fp32 data; the bin lookup of eval_G's fp32 path from the composed
+// coefficients).
+template <int GEO>
+__device__ __forceinline__ double eval_G_lk(const double2* __restrict__ t, double pi, const LineLk& k, int i,
+                                            const Lookup& L, float hi) {
+    const float num = i == 0 ? k.n0 : __builtin_fmaf((float)i, k.C0, k.n0);
+    float q;
+    if (GEO == GEO_PARA) {
+        q = num;
+    } else {
+        const float den = i == 0 ? k.d0 : __builtin_fmaf((float)i, k.C2, k.d0);
+        q = num * __builtin_amdgcn_rcpf(den);
+        if (GEO == GEO_EA) q = __builtin_fmaf(atanf(q), L.ea_sc, L.ea_off);
+        if (GEO == GEO_EAP) {
+            const float tc = __builtin_amdgcn_fmed3f(q, -L.T, L.T);
+            const float s2 = tc * tc;
+            float z = L.c[5];
+#pragma unroll
+            for (int j = 4; j >= 0; j--) z = __builtin_fmaf(z, s2, L.c[j]);
+            q = __builtin_fmaf(tc * z, L.ea_sc, L.ea_off);
+        }
+    }
+    const double2 c = t[table_index(q, hi)];
+    return __builtin_fma(pi, c.y, c.x);
+}
+
 // LDS bytes of one adj_block.
 __host__ __device__ inline size_t adj_block_lds(int D, int L) {
-    return (size_t)2 * (D + 2) * 16 + (size_t)2 * L * LC_WORDS * 8;
+    return (size_t)2 * (D + 2) * 16 + (size_t)2 * L * LC_WORDS * 8 + (size_t)2 * L * 16;
 }
 
 // One bin -> pixel work item: batch element b, sweep axis `mode`, image lines
@@ -701,6 +751,8 @@ __device__ __forceinline__ void adj_block(const DevPlan& P, const double2* __res
 
     double2* stab = reinterpret_cast<double2*>(smem);          // [2][tabn]
     double* scf = reinterpret_cast<double*>(stab + 2 * tabn);  // [2][L][LC_WORDS]
+    float4* slk = reinterpret_cast<float4*>(scf + 2 * L * LC_WORDS);  // [2][L] composed lookups (fp32 data)
+    constexpr bool LK = sizeof(T) == 4;
 
     const double2* tb_b = tabs + (size_t)b * P.V * tabn;
     // Asynchronous copy of view v's table into LDS buffer `buf`: wave w copies
@@ -721,6 +773,12 @@ __device__ __forceinline__ void adj_block(const DevPlan& P, const double2* __res
     auto coef_word = [&](int v) {
         return cvalid ? reinterpret_cast<const double*>(P.lines + (size_t)v * P.max_lines + cl)[tid % LC_WORDS] : 0.0;
     };
+    // composed lookup of tile line `tid`, held by thread tid < L (fp32 data)
+    const bool athread = LK && tid < L;
+    const int al = tile * L + tid;
+    auto lk_word = [&](int v) {
+        return athread && al < nlines ? P.adj_lk[(size_t)v * P.max_lines + al] : make_float4(0.f, 0.f, 0.f, 1.f);
+    };
 
     double acc[LPW][NG];
 #pragma unroll
@@ -732,6 +790,7 @@ __device__ __forceinline__ void adj_block(const DevPlan& P, const double2* __res
 
     table_to_lds(vlist[vb], 0);
     if (cthread) scf[tid] = coef_word(vlist[vb]);
+    if (athread) slk[tid] = lk_word(vlist[vb]);
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     __syncthreads();
 
@@ -739,13 +798,16 @@ __device__ __forceinline__ void adj_block(const DevPlan& P, const double2* __res
         const int cur = (iv - vb) & 1;
         const int v = vlist[iv];
         double cnext = 0.0;
+        float4 anext = make_float4(0.f, 0.f, 0.f, 1.f);
         if (iv + 1 < nv) {
             const int vn = vlist[iv + 1];
             table_to_lds(vn, cur ^ 1);
             cnext = coef_word(vn);
+            if (athread) anext = lk_word(vn);
         }
         const double2* t = stab + cur * tabn;
         const double* cf = scf + cur * L * LC_WORDS;
+        const float4* clk = slk + cur * L;
         const Lookup lu = view_lookup(P, v);
         double gx = 0.0;  // EXACT: G at edge n of line lx (lanes >= LPW: unused)
         if (EXACT) {
@@ -761,9 +823,21 @@ __device__ __forceinline__ void adj_block(const DevPlan& P, const double2* __res
             // Every slot is evaluated, also past the last pixel edge (clamped table
             // index, never stored): no branch between the independent evaluations,
             // so their LDS reads overlap.
+            // G at pixel edge lane + 64 i of this line, called for i = 0, 1, 2, ... in order
+            LineLk lk{};
+            if (LK) lk = line_lk(a, s, clk[lq], lane);
+            double pir = lk.pi0;  // axis coordinate of edge lane + 64 i (fp32 data)
+            auto G_at = [&](int i) {
+                if constexpr (LK) {
+                    if (i > 0) pir += lk.s64;
+                    return eval_G_lk<GEO>(t, pir, lk, i, lu, hi);
+                } else {
+                    return eval_G<GEO, true>(t, a, s, lane + 64 * i, lu, hi);
+                }
+            };
             if (!WEIGHTED) {
 #pragma unroll
-                for (int i = 0; i < NG; i++) acc[q][i] += eval_G<GEO, sizeof(T) == 8>(t, a, s, lane + 64 * i, lu, hi);
+                for (int i = 0; i < NG; i++) acc[q][i] += G_at(i);
             } else {
                 const double w0 = cf[lq * LC_WORDS + LC_W0], w1 = cf[lq * LC_WORDS + LC_W1];
                 const double w2 = cf[lq * LC_WORDS + LC_W2];
@@ -786,7 +860,7 @@ __device__ __forceinline__ void adj_block(const DevPlan& P, const double2* __res
                 double gprev = 0.0, rprev = 0.0;
 #pragma unroll
                 for (int i = 0; i < NG; i++) {
-                    const double gi = eval_G<GEO, sizeof(T) == 8>(t, a, s, lane + 64 * i, lu, hi);
+                    const double gi = G_at(i);
                     const double ri = rot_next_d(gi);
                     if (i > 0) {
                         const double right = lane == 63 ? ri : rprev;
@@ -807,6 +881,7 @@ __device__ __forceinline__ void adj_block(const DevPlan& P, const double2* __res
             }
         }
         if (iv + 1 < nv && cthread) scf[(cur ^ 1) * L * LC_WORDS + tid] = cnext;
+        if (iv + 1 < nv && athread) slk[(cur ^ 1) * L + tid] = anext;
         asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
         __syncthreads();
     }

@@ -21,6 +21,7 @@ struct DevPlan {
     const double* taps;       // [2D-1]
     const double* preweight;  // [D]
     const double* lookup_d;   // [V][6] lookup coefficients in fp64 (used for fp64 data)
+    const float4* adj_lk;     // [V][max_lines] lookup composed with the line's pixel-edge map (HostPlan)
     int nx, ny;
     int V, D, W, H, max_lines;
     int scan_type;

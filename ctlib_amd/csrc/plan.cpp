@@ -140,6 +140,7 @@ int build_plan(const Geom& g, HostPlan* P, std::string* why) {
     P->post_prj.assign((size_t)V * D, 0.0);
     P->post_bprj_t.assign((size_t)V * D, 1.0);
     P->lines.assign((size_t)V * P->max_lines, LineCoef{});
+    P->adj_lk.assign((size_t)V * P->max_lines * 4, 0.0f);
     P->lookup_d.assign((size_t)V * 6, 0.0);
     P->xviews.clear();
     P->yviews.clear();
@@ -301,6 +302,13 @@ int build_plan(const Geom& g, HostPlan* P, std::string* why) {
             c.B = 1.0 - c.a / c.s;
             c.inv_s = 1.0 / c.s;
             lc[l] = c;
+            // the view's lookup (lkd) at pixel edge p of this line, lk(a + s p) = (c0 p + c1) / (c2 p + c3),
+            // composed in fp64; c0 and c2 are stored times 64 (the kernels step p in 64s)
+            float* ck = &P->adj_lk[((size_t)v * P->max_lines + l) * 4];
+            ck[0] = (float)(64.0 * lkd[0] * c.s);
+            ck[1] = (float)(lkd[0] * c.a + lkd[1]);
+            ck[2] = (float)(64.0 * lkd[2] * c.s);
+            ck[3] = (float)(lkd[2] * c.a + lkd[3]);
         }
     }
 

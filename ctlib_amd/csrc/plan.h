@@ -55,6 +55,10 @@ struct HostPlan {
     std::vector<double> post_bprj_t;     // [V][D]   backprojection_t factor: 1, or dImg/c_k for fan-EA
     std::vector<LineCoef> lines;         // [V][max_lines]
     std::vector<double> lookup_d;        // [V][6] ViewParams lk0..lk3, ea_sc, ea_off in fp64
+    // [V][max_lines][4] the view's lookup composed with the line's pixel-edge map, fp32:
+    // {64*lk0*s, lk0*a + lk1, 64*lk2*s, lk2*a + lk3}, so that at pixel edge p the lookup
+    // argument is (c0*p/64 + c1) / (c2*p/64 + c3) (the bin-to-pixel kernels' fp32 path)
+    std::vector<float> adj_lk;
     std::vector<int> xviews, yviews;     // view indices per sweep axis, ascending
     std::vector<double> ramp_taps;       // [2D-1] RL taps h(-(D-1)..D-1)
     std::vector<double> ramp_preweight;  // [D] fbp pre-weight (output bin order)

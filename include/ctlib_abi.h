@@ -149,7 +149,9 @@ int ctlib_plan_view_info(const ctlib_plan* plan, int view, int* xmode, int* rev)
  * view list int32, 6 Y-axis view list int32, 7 RL taps [2D-1] f64, 8 fbp
  * pre-weight [D] f64, 9 fan-EA arctangent polynomial as 8 f32: c0..c5 of
  * atan(t) ~ t (c0 + c1 t^2 + ... + c5 t^10) on |t| <= T, then T, then 1 if the
- * kernels use it (else they call atanf).  With dst == NULL only *bytes is set. */
+ * kernels use it (else they call atanf), 10 the view lookup composed with each
+ * line's pixel-edge map [V][max(H,W)][4] f32 {64 lk0 s, lk0 a + lk1, 64 lk2 s,
+ * lk2 a + lk3}.  With dst == NULL only *bytes is set. */
 int ctlib_plan_export(const ctlib_geom* geom, int table, void* dst, size_t* bytes);
 
 /* Heterogeneous batch: slices of different geometry (image size, views, bins,

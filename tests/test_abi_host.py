@@ -129,6 +129,33 @@ def test_kernel_model_parallel_rows_at_full_config0():
     assert np.linalg.norm(got - ref) / np.linalg.norm(ref) < 1e-13
 
 
+@pytest.mark.parametrize("st", [0, 1, 2])
+def test_composed_lookup_matches_view_lookup(st):
+    """Plan table adj_lk - each view's bin lookup composed with each line's pixel-edge map, which the
+    fp32 bin->pixel kernels evaluate at pixel edge p as (c0 p/64 + c1) / (c2 p/64 + c3) - gives the
+    lookup argument of the view's own lookup at a + s p; for fan-ED and parallel beam that lookup
+    puts the bin edges e_k of the (oracle-checked) edge table at k + 1."""
+    o = small(st, V=30, D=19, W=14, H=11, Ang0=0.25, bs=-0.03)
+    g = abi.parse_options(o)
+    V, D = g.views, g.dets
+    nmax = max(g.width, g.height)
+    vp = abi.plan_export(o, "views")
+    lines = abi.plan_export(o, "lines").reshape(V, nmax)
+    lk = abi.plan_export(o, "adj_lk").reshape(V, nmax, 4).astype(np.float64)
+    edges = abi.plan_export(o, "edges").reshape(V, D + 1)
+    for v in range(V):
+        c = vp[v]["lk"].astype(np.float64)
+        view_lk = lambda e: (c[0] * e + c[1]) / (c[2] * e + c[3])
+        if st != 1:
+            np.testing.assert_allclose(view_lk(edges[v]), np.arange(1, D + 2), rtol=0, atol=2e-4)
+        p = np.arange(vp[v]["npix"] + 1, dtype=np.float64)
+        for l in range(vp[v]["nlines"]):
+            want = view_lk(lines[v, l]["a"] + lines[v, l]["s"] * p)
+            k = lk[v, l]
+            got = (k[0] * p / 64 + k[1]) / (k[2] * p / 64 + k[3])
+            np.testing.assert_allclose(got, want, rtol=0, atol=2e-6 * max(1.0, np.max(np.abs(want))))
+
+
 @pytest.mark.parametrize("dDet, expect_poly", [(0.03, True), (0.0011, True), (0.0005, True), (0.0016, False)])
 def test_fan_ea_atan_polynomial(dDet, expect_poly):
     """The fan-EA lookup replaces atanf by an odd polynomial on |t| <= T: T must cover every bin edge's
