@@ -58,7 +58,12 @@ __global__ __launch_bounds__(256) void k_combine_parts(T* __restrict__ out, cons
 // groups, all evaluated before the stores; the right edge comes from the
 // neighbouring lane), so stores are row-contiguous.  Y-axis views: each lane
 // owns a column and walks its pixels bottom to top, SV_UNROLL at a time, so
-// the 64 lanes of a wave store 64 adjacent columns of one row.
+// the 64 lanes of a wave store 64 adjacent columns of one row.  The kernel is
+// VALU-bound (95 % issue at B=8, 736 views, 512^2: its 6.2 GB of stores alone
+// would take ~1.07 ms at the 5.7 TB/s a plain store kernel reaches), so the
+// fp32 path evaluates the composed lookup (plan table adj_lk) and fp32 pixel
+// weights, and steps each column's fp64 edge coordinate and float pixel
+// counter by adds.
 constexpr int SV_UNROLL = 8;
 
 template <typename T, int GEO, int NPE>
@@ -86,6 +91,94 @@ __global__ __launch_bounds__(256) void k_bprj_sv(DevPlan P, const double2* __res
         const double d = __builtin_fma(u, u, c.w2);
         return c.inv_s * wscale * (sizeof(T) == 8 ? rcp_refined(d) : (double)__builtin_amdgcn_rcpf((float)d));
     };
+    if constexpr (sizeof(T) == 4) {
+        // fp32 data: the lookup composed with each line's pixel-edge map (plan table adj_lk) and the
+        // pixel weight in fp32, as in the bin->pixel kernels; the fp64 edge differences are rounded to
+        // fp32 before the weight is applied.
+        const float4* alk = P.adj_lk + (size_t)v * P.max_lines;
+        if (vp.xmode) {
+            for (int r = wave; r < H; r += NT / 64) {
+                const LineCoef c = lc[r];
+                const LineLk lk = line_lk(c.a, c.s, alk[r], lane);
+                T* dst = img + (size_t)r * W;
+                double g[NPE];
+                double pir = lk.pi0;
+#pragma unroll
+                for (int i = 0; i < NPE; i++) {
+                    if (i > 0) pir += lk.s64;
+                    g[i] = eval_G_lk<GEO>(t, pir, lk, i, lu, hi);
+                }
+                const float w1f = (float)c.w1, w2f = (float)c.w2, wsf = (float)(c.inv_s * wscale);
+                const float u0 = __builtin_fmaf(w1f, (float)lane, (float)c.w0), w1x64 = 64.0f * w1f;
+                // right edge of pixel lane + 64 i: edge lane + 1 of group i, for lane 63 edge 0 of
+                // group i + 1 - both from the groups' rotations by one lane
+                const bool last = lane == 63;
+                double rcur = rot_next_d(g[0]);
+#pragma unroll
+                for (int i = 0; i < NPE; i++) {
+                    // lane 63 of the last group is past the row: 64 * NPE > W
+                    double right = rcur;
+                    if (i + 1 < NPE) {
+                        const double rnext = rot_next_d(g[i + 1]);
+                        if (last) right = rnext;
+                        rcur = rnext;
+                    }
+                    const int p = lane + 64 * i;
+                    const float u = i == 0 ? u0 : __builtin_fmaf((float)i, w1x64, u0);
+                    const float w = wsf * __builtin_amdgcn_rcpf(__builtin_fmaf(u, u, w2f));
+                    if (p < W) dst[p] = (float)(right - g[i]) * w;
+                }
+            }
+        } else {
+            for (int c0 = wave * 64; c0 < W; c0 += NT) {
+                const int col = c0 + lane;
+                const bool ok = col < W;
+                const LineCoef c = lc[ok ? col : W - 1];
+                const float4 ck = alk[ok ? col : W - 1];
+                // edge p of the column: axis coordinate a + s p (stepped in fp64), lookup argument
+                // (c0 p + c1) / (c2 p + c3) with p held exactly as a float counter
+                const float cx = ck.x * (1.0f / 64.0f), cz = ck.z * (1.0f / 64.0f);
+                const float w0f = (float)c.w0, w1f = (float)c.w1, w2f = (float)c.w2;
+                const float wsf = (float)(c.inv_s * wscale);
+                auto G_edge = [&](double pi, float pf) {
+                    const float num = __builtin_fmaf(cx, pf, ck.y);
+                    float q;
+                    if (GEO == GEO_PARA) {
+                        q = num;
+                    } else {
+                        q = num * __builtin_amdgcn_rcpf(__builtin_fmaf(cz, pf, ck.w));
+                        if (GEO == GEO_EA) q = __builtin_fmaf(atanf(q), lu.ea_sc, lu.ea_off);
+                        if (GEO == GEO_EAP) {
+                            const float tc = __builtin_amdgcn_fmed3f(q, -lu.T, lu.T);
+                            const float s2 = tc * tc;
+                            float z = lu.c[5];
+#pragma unroll
+                            for (int j = 4; j >= 0; j--) z = __builtin_fmaf(z, s2, lu.c[j]);
+                            q = __builtin_fmaf(tc * z, lu.ea_sc, lu.ea_off);
+                        }
+                    }
+                    const double2 e = t[table_index(q, hi)];
+                    return __builtin_fma(pi, e.y, e.x);
+                };
+                double pi = c.a;
+                float pf = 0.0f;
+                double gprev = G_edge(pi, pf);
+                T* dst = img + (size_t)(H - 1) * W + col;
+#pragma unroll 8
+                for (int i = 0; i < H; i++) {  // pixel i between edges i and i + 1
+                    const float pix = pf;      // == i
+                    pi += c.s;
+                    pf += 1.0f;
+                    const double gi = G_edge(pi, pf);
+                    const float u = __builtin_fmaf(w1f, pix, w0f);
+                    const float w = wsf * __builtin_amdgcn_rcpf(__builtin_fmaf(u, u, w2f));
+                    if (ok) dst[-(ptrdiff_t)i * W] = (float)(gi - gprev) * w;
+                    gprev = gi;
+                }
+            }
+        }
+        return;
+    }
     if (vp.xmode) {
         for (int r = wave; r < H; r += NT / 64) {
             const LineCoef c = lc[r];
