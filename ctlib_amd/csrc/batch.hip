@@ -58,22 +58,33 @@ struct Item {
 };
 
 
-template <typename T>
+// Forward of one F item.  Edge slots per view: 12 (D < 768), 20 (D < 1280) or 24 (D < 1536); 512,
+// 1024 or 1536 bins: 8, 16 or 24 slots and the last edge packed.  Pixels per thread M = 2 (lines
+// <= 512) or, in a batch with a slice beyond 512 x 512 (LARGE), M = 4 for every slice (lines <= 1024).
+template <typename T, bool LARGE>
 __device__ __forceinline__ void run_fwd(const SliceDev& sd, int mode, int grp, unsigned char* smem) {
     const T* src = (const T*)(mode == MODE_Y ? sd.xT : sd.x);
     T* y = (T*)sd.y;
     const T* m = (const T*)sd.m;
-    // edge slots per view: 12 (D < 768) or 20 (D < 1280); 512 or 1024 bins: 8 or 16 slots and the last
-    // edge packed; M = 2 (lines <= 512)
+    constexpr int M = LARGE ? 4 : 2;
     switch (sd.fcls) {
-    case 0: fwd_block<T, 1, 12, 2, false, true>(sd.P, src, y, 0, mode, grp, sd.P.post_prj, m, sd.tabs, smem); break;
+    case 0: fwd_block<T, 1, 12, M, false, true>(sd.P, src, y, 0, mode, grp, sd.P.post_prj, m, sd.tabs, smem); break;
     case 2:
-        fwd_block<T, 1, 8, 2, false, true, false, true>(sd.P, src, y, 0, mode, grp, sd.P.post_prj, m, sd.tabs, smem);
+        fwd_block<T, 1, 8, M, false, true, false, true>(sd.P, src, y, 0, mode, grp, sd.P.post_prj, m, sd.tabs, smem);
         break;
     case 3:
-        fwd_block<T, 1, 16, 2, false, true, false, true>(sd.P, src, y, 0, mode, grp, sd.P.post_prj, m, sd.tabs, smem);
+        fwd_block<T, 1, 16, M, false, true, false, true>(sd.P, src, y, 0, mode, grp, sd.P.post_prj, m, sd.tabs, smem);
         break;
-    default: fwd_block<T, 1, 20, 2, false, true>(sd.P, src, y, 0, mode, grp, sd.P.post_prj, m, sd.tabs, smem); break;
+    case 4:
+        if constexpr (LARGE)
+            fwd_block<T, 1, 24, M, false, true>(sd.P, src, y, 0, mode, grp, sd.P.post_prj, m, sd.tabs, smem);
+        break;
+    case 5:
+        if constexpr (LARGE)
+            fwd_block<T, 1, 24, M, false, true, false, true>(sd.P, src, y, 0, mode, grp, sd.P.post_prj, m, sd.tabs,
+                                                             smem);
+        break;
+    default: fwd_block<T, 1, 20, M, false, true>(sd.P, src, y, 0, mode, grp, sd.P.post_prj, m, sd.tabs, smem); break;
     }
 }
 
@@ -92,14 +103,20 @@ __device__ __forceinline__ void run_adj_geo(const SliceDev& sd, int mode, int ti
 }
 
 // Line classes: 0 lines <= 256 px, 1 lines <= 512 px; all lines exactly 2: 256, 3: 512, 4: 384 px
-// (the last pixel edge evaluated packed).
-template <typename T>
+// (the last pixel edge evaluated packed); LARGE batches also 5: lines <= 1024 px, 6: exactly 1024 px.
+template <typename T, bool LARGE>
 __device__ __forceinline__ void run_adj(const SliceDev& sd, int mode, int tile, unsigned char* smem) {
     switch (sd.acls >> 2) {
     case 0: run_adj_geo<T, 8, 5, false>(sd, mode, tile, smem); break;
     case 2: run_adj_geo<T, 8, 5, true>(sd, mode, tile, smem); break;
     case 3: run_adj_geo<T, 4, 9, true>(sd, mode, tile, smem); break;
     case 4: run_adj_geo<T, 4, 7, true>(sd, mode, tile, smem); break;
+    case 5:
+        if constexpr (LARGE) run_adj_geo<T, 2, 17, false>(sd, mode, tile, smem);
+        break;
+    case 6:
+        if constexpr (LARGE) run_adj_geo<T, 2, 17, true>(sd, mode, tile, smem);
+        break;
     default: run_adj_geo<T, 4, 9, false>(sd, mode, tile, smem); break;
     }
 }
@@ -125,7 +142,7 @@ __global__ void k_batch_reset(unsigned int* ctr, int n) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) ctr[i] = 0u;
 }
 
-template <typename T>
+template <typename T, bool LARGE>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_batch(const SliceDev* __restrict__ slices, const Item* __restrict__ items,
                                                int nitems, unsigned int* __restrict__ ctr, int nslices) {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -177,11 +194,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
             done = tdone + I.slice;
             break;
         case IT_F:
-            run_fwd<T>(sd, I.mode, I.idx, smem);
+            run_fwd<T, LARGE>(sd, I.mode, I.idx, smem);
             done = fdone + I.slice;
             break;
         case IT_A:
-            run_adj<T>(sd, I.mode, I.idx, smem);
+            run_adj<T, LARGE>(sd, I.mode, I.idx, smem);
             done = adone + I.slice;
             break;
         default:
@@ -224,6 +241,7 @@ struct ctlib_batch {
     int order = 0;
     int num_cu = 256;
     size_t lds = 0;
+    bool large = false;  // a slice beyond 512 x 512 or 1279 bins: the LARGE kernel variant
     bool bound = false;
 };
 
@@ -297,8 +315,9 @@ int ctlib_batch_create(const ctlib_geom* geoms, int nslices, int device, int dty
     size_t off = 0;
     for (int s = 0; s < nslices; s++) {
         const ctlib_geom& g = geoms[s];
-        if (g.width > 512 || g.height > 512 || g.dets > 1279)
-            return cleanup(CTLIB_ERR_GEOMETRY, "batched slices support width, height <= 512 and dets <= 1279");
+        if (g.width > 1024 || g.height > 1024 || g.dets > 1536)
+            return cleanup(CTLIB_ERR_GEOMETRY, "batched slices support width, height <= 1024 and dets <= 1536");
+        if (g.width > 512 || g.height > 512 || g.dets > 1279) b->large = true;
         ctlib_plan* p = nullptr;
         int rc = ctlib_plan_create(&g, device, &p);
         if (rc) return cleanup(rc, std::string("slice ") + std::to_string(s) + ": " + ctlib_last_error());
@@ -308,13 +327,20 @@ int ctlib_batch_create(const ctlib_geom* geoms, int nslices, int device, int dty
         const DevPlan& P = sd.P;
         const int ne = (P.D + 1 + 63) / 64;
         const int nmax = std::max(P.nx ? P.W : 0, P.ny ? P.H : 0);
-        sd.fcls = P.D == 512 ? 2 : P.D == 1024 ? 3 : ne <= 12 ? 0 : 1;
+        // forward class (run_fwd; 4 and 5 occur only in LARGE batches)
+        sd.fcls = P.D == 512 ? 2 : P.D == 1024 ? 3 : P.D == 1536 ? 5 : ne <= 12 ? 0 : ne <= 20 ? 1 : 4;
         const int npe = (nmax + 1 + 63) / 64;
         // every line (rows of X views, columns of Y views) exactly n pixels
         auto all_lines = [&](int n) { return (P.nx == 0 || P.W == n) && (P.ny == 0 || P.H == n); };
-        const int lcls = all_lines(256) ? 2 : all_lines(512) ? 3 : all_lines(384) ? 4 : npe <= 5 ? 0 : 1;
+        const int lcls = all_lines(256)    ? 2
+                         : all_lines(512)  ? 3
+                         : all_lines(384)  ? 4
+                         : all_lines(1024) ? 6
+                         : npe <= 5        ? 0
+                         : npe <= 9        ? 1
+                                           : 5;
         sd.acls = lcls * 4 + (P.scan_type == GEO_EA && P.ea_poly ? GEO_EAP : P.scan_type);
-        const int L = lcls == 0 || lcls == 2 ? 32 : 16;
+        const int L = lcls == 0 || lcls == 2 ? 32 : lcls >= 5 ? 8 : 16;
         sd.tileL = L;
         sd.ngx = (P.nx + BG - 1) / BG;
         sd.ngy = (P.ny + BG - 1) / BG;
@@ -418,20 +444,18 @@ int ctlib_batch_run(const ctlib_batch* b, void* stream) {
     hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(256), 0, s, b->d_ctr, b->nctr);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) {
-        auto kern = b->dtype == CTLIB_F64 ? (const void*)k_batch<double> : (const void*)k_batch<float>;
-        if (b->lds > 64 * 1024)
-            e = hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)b->lds);
-        const int grid = b->num_cu * b->blocks_per_cu;
-        const int nitems = (int)b->items.size();
-        if (e == hipSuccess) {
-            if (b->dtype == CTLIB_F64)
-                hipLaunchKernelGGL(k_batch<double>, dim3(grid), dim3(256), b->lds, s, b->d_slices, b->d_items, nitems,
-                                   b->d_ctr, b->nslices);
-            else
-                hipLaunchKernelGGL(k_batch<float>, dim3(grid), dim3(256), b->lds, s, b->d_slices, b->d_items, nitems,
-                                   b->d_ctr, b->nslices);
+        auto kernel = [&](auto kern) {
+            if (b->lds > 64 * 1024)
+                e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)b->lds);
+            if (e != hipSuccess) return;
+            hipLaunchKernelGGL(kern, dim3(b->num_cu * b->blocks_per_cu), dim3(256), b->lds, s, b->d_slices,
+                               b->d_items, (int)b->items.size(), b->d_ctr, b->nslices);
             e = hipGetLastError();
-        }
+        };
+        if (b->dtype == CTLIB_F64)
+            b->large ? kernel(k_batch<double, true>) : kernel(k_batch<double, false>);
+        else
+            b->large ? kernel(k_batch<float, true>) : kernel(k_batch<float, false>);
     }
     roctxRangePop();
     if (prev >= 0) (void)hipSetDevice(prev);

@@ -161,7 +161,9 @@ int ctlib_plan_export(const ctlib_geom* geom, int table, void* dst, size_t* byte
  * projection groups, projection_t tiles, axis combines) are dequeued from one
  * atomic counter; per-slice counters order the phases.  ctlib_batch_run
  * allocates nothing, synchronises nothing and records no memset, so it may be
- * captured in a HIP graph.  Slices: width, height <= 512, dets <= 1279.
+ * captured in a HIP graph.  Slices: width, height <= 1024, dets <= 1536 (a batch
+ * with any slice beyond 512 x 512 or 1279 bins runs a kernel variant whose
+ * line tables hold up to 1024 pixels, for every slice of that batch).
  * No reference counterpart (the reference runs one geometry per call). */
 typedef struct ctlib_batch ctlib_batch;
 int ctlib_batch_create(const ctlib_geom* geoms, int nslices, int device, int dtype, ctlib_batch** out);

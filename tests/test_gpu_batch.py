@@ -113,6 +113,48 @@ def test_batch_config3_against_plain(dev):
         assert rel(grads[s].cpu().numpy()[:, 0], g_or) < 1e-5, s
 
 
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_batch_with_large_slices(dev, dtype):
+    """A batch holding slices beyond 512 x 512 / 1279 bins (config 4's 1024^2, 1536-bin fan-ED geometry
+    with fewer views, a 768 x 640 fan-EA slice with 1300 bins, a 1000 x 1000 parallel slice) together
+    with small ones runs the large-line kernel variant for every slice and equals the plain per-slice
+    calls; the small slices also match the oracle."""
+    o = [
+        phantom.fan_options(180, 1536, 1024, 1024, 0.0033, 0.0036, 5.95, 4.906, 0),
+        phantom.fan_options(90, 1300, 768, 640, 0.004, 0.0006, 5.95, 4.906, 1, binshift=0.0002),
+        phantom.para_options(60, 1100, 1000, 1000, 0.002, 0.002, binshift=0.001),
+        phantom.fan_options(60, 47, 32, 32, 0.05, 0.06, 3.0, 2.0, 0),
+        phantom.para_options(36, 53, 40, 40, 0.03, 0.03),
+    ]
+    rng = np.random.default_rng(78)
+    xs, ms = [], []
+    for oo in o:
+        g = O.parse(oo)
+        xs.append(torch.from_numpy(rng.random((1, 1, g["height"], g["width"])).astype(dtype)).to(dev))
+        ms.append(torch.from_numpy(rng.random((1, 1, g["views"], g["dets"])).astype(dtype)).to(dev))
+    print(f"[case] batch large slices {dtype.__name__}: build", flush=True)
+    tdt = torch.float32 if dtype == np.float32 else torch.float64
+    hb = batch.HeteroBatch([torch.from_numpy(oo) for oo in o], device=dev, dtype=tdt, graph=False)
+    print(f"[case] batch large slices: run ({hb.info()['items']} items)", flush=True)
+    ys, grads = hb.run(xs, ms)
+    hb.check()
+    # same arithmetic, different summation order (the plain calls split lines into chunks): fp64
+    # rounding over 1024-pixel lines reaches ~1e-12
+    tol_plain = 1e-6 if dtype == np.float32 else 1e-10
+    for s, oo in enumerate(o):
+        y_ref, g_ref = ctlib_amd.residual_step(xs[s], ms[s], torch.from_numpy(oo))
+        assert rel(ys[s].cpu().numpy(), y_ref.cpu().numpy()) < tol_plain, s
+        assert rel(grads[s].cpu().numpy(), g_ref.cpu().numpy()) < tol_plain, s
+    for s in (3, 4):
+        o64 = o[s].astype(np.float64)
+        x64 = xs[s].cpu().numpy()[:, 0].astype(np.float64)
+        y_or = O.apply("projection", o64, x64)
+        g_or = O.apply("projection_t", o64, y_or - ms[s].cpu().numpy()[:, 0].astype(np.float64))
+        tol = 1e-5 if dtype == np.float32 else 1e-7
+        assert rel(ys[s].cpu().numpy()[:, 0], y_or) < tol, s
+        assert rel(grads[s].cpu().numpy()[:, 0], g_or) < tol, s
+
+
 def test_tuner_roundtrip(dev, tmp_path, monkeypatch):
     monkeypatch.setenv("CTLIB_AMD_TUNE_CACHE", str(tmp_path / "tune.json"))
     o = phantom.fan_options(96, 128, 64, 64, 0.05, 0.06, 5.95, 4.906, 0)
