@@ -605,13 +605,13 @@ def main():
     per_launch_ms = st["ms"] / max(st["launches"], 1)
     per_launch_bytes = st["algo_bytes"] / max(st["launches"], 1)
     achieved = per_launch_bytes / (per_launch_ms * 1e-3) / 1e9
-    kname = "k_fwd" if k == "fwd" else "k_adj"
+    kname = "k_fwd_lt" if k == "fwd" else "k_adj"
     traffic = measured_traffic(kname) if B == 32 else None
     ctr = issue_counters(kname) if B == 32 else None
     # The DD kernels reuse image lines and bin tables from LDS, so what binds them is on chip:
     # the VALU issue rate (instructions counted by rocprofv3, time measured here) and the LDS.
     # The HBM figures are reported beside it.
-    roof = {"kernel": "k_fwd (line->bin)" if k == "fwd" else "k_adj (bin->pixel)",
+    roof = {"kernel": "k_fwd_lt (line->bin over prebuilt line tables)" if k == "fwd" else "k_adj (bin->pixel)",
             "kernel_ms": round(per_launch_ms, 4)}
     if ctr:
         valu = ctr["SQ_INSTS_VALU"] / (per_launch_ms * 1e-3) / 1e9

@@ -24,7 +24,8 @@ EXPORTS = ("ctlib_parse_options", "ctlib_plan_create", "ctlib_plan_destroy", "ct
            "ctlib_plan_export", "ctlib_timing_enable", "ctlib_timing_collect", "ctlib_last_error",
            "ctlib_status_string", "ctlib_batch_create", "ctlib_batch_destroy", "ctlib_batch_workspace_bytes",
            "ctlib_batch_set_schedule", "ctlib_batch_bind", "ctlib_batch_run", "ctlib_batch_info",
-           "ctlib_batch_last_error", "ctlib_plan_set_tuning", "ctlib_plan_set_chunks", "ctlib_batch_check",
+           "ctlib_batch_last_error", "ctlib_plan_set_tuning", "ctlib_plan_set_chunks",
+           "ctlib_plan_set_line_tables", "ctlib_batch_check",
            "ctlib_laplacian", "ctlib_laplacian_workspace_bytes", "ctlib_run_noalloc")
 
 

@@ -36,6 +36,7 @@ struct ctlib_plan {
     std::atomic<int> fwd_G{0};    // autotuned views per line->bin block (0 = automatic)
     std::atomic<int> adj_nch{0};  // view chunks of the bin->pixel kernel (0 = automatic)
     std::atomic<int> fwd_nch{0};  // line chunks of the line->bin kernel (0 = automatic)
+    std::atomic<int> fwd_lt{0};   // line->bin line tables: 0 prebuilt in the workspace, 1 per block
 };
 
 namespace ctlib {
@@ -86,12 +87,22 @@ struct DeviceGuard {
 
 // The plan's launch-shape overrides as one call sees them.
 struct Tuning {
-    int fwd_G, adj_nch, fwd_nch;
+    int fwd_G, adj_nch, fwd_nch, fwd_lt;
 };
 
 Tuning snap(const ctlib_plan* p) {
     return Tuning{p->fwd_G.load(std::memory_order_relaxed), p->adj_nch.load(std::memory_order_relaxed),
-                  p->fwd_nch.load(std::memory_order_relaxed)};
+                  p->fwd_nch.load(std::memory_order_relaxed), p->fwd_lt.load(std::memory_order_relaxed)};
+}
+
+// Views per block of the line -> bin launch.  Weighted: 4.  Unweighted, with two views per
+// wave possible (<= 12 edge slots of 64 per view): `tuned` (4 or 8) if set, else 4 over
+// prebuilt line tables and 8 over tables built per block (measured at fan-ED 512^2,
+// 736 views, B=32: 2.27 ms at 4 vs 2.40 at 8 prebuilt; 2.84 at 8 vs 3.40 at 4 per block).
+int fwd_views(const DevPlan& P, bool weighted, int tuned, bool prebuilt) {
+    if (weighted || (P.D + 1 + 63) / 64 > 12) return 4;
+    if (tuned == 4 || tuned == 8) return tuned;
+    return prebuilt ? 4 : 8;
 }
 
 // ---- scratch of ctlib_run_noalloc: one fixed device array per GPU, no allocation per call.
@@ -178,7 +189,7 @@ int fwd_chunks(const ctlib_plan* p, const Tuning& T, int64_t B, bool weighted) {
     if (P.ny && P.W < lmin) lmin = P.W;
     if (lmin == 1 << 30) return 1;
     if (T.fwd_nch > 0) return T.fwd_nch < lmin ? T.fwd_nch : lmin;
-    const int G = weighted ? 4 : ((P.D + 1 + 63) / 64 <= 12 && T.fwd_G != 4 ? 8 : 4);
+    const int G = fwd_views(P, weighted, T.fwd_G, !weighted && T.fwd_lt == 0);
     const int64_t items = B * ((P.nx + G - 1) / G + (P.ny + G - 1) / G);
     int64_t nch = (4096 + items - 1) / items;
     if (nch > 16) nch = 16;
@@ -188,11 +199,14 @@ int fwd_chunks(const ctlib_plan* p, const Tuning& T, int64_t B, bool weighted) {
 
 // Scratch layout of every op (byte offsets into ws).
 struct WsLayout {
-    size_t ylines = 0, tables = 0, yt = 0, xp = 0, sp = 0, rows = 0, total = 0;
-    size_t n_ylines = 0, n_tables = 0, n_yt = 0, n_xp = 0, n_sp = 0, n_rows = 0;
+    size_t ylines = 0, tables = 0, yt = 0, xp = 0, sp = 0, rows = 0, ltab = 0, total = 0;
+    size_t n_ylines = 0, n_tables = 0, n_yt = 0, n_xp = 0, n_sp = 0, n_rows = 0, n_ltab = 0;
     int nch = 1;   // view chunks of the bin -> pixel launch (partials in xp / yt when > 1)
     int fnch = 1;  // line chunks of the line -> bin launch (partial sinograms in sp when > 1)
     int fwd_G = 0; // views per line -> bin block override (Tuning::fwd_G)
+
+    // prebuilt line tables of the unweighted line -> bin launch (null: built per block)
+    double2* line_tables(char* ws) const { return n_ltab ? reinterpret_cast<double2*>(ws + ltab) : nullptr; }
 };
 
 WsLayout layout(const ctlib_plan* p, const Tuning& T, int op, int64_t B, int dtype) {
@@ -217,6 +231,7 @@ WsLayout layout(const ctlib_plan* p, const Tuning& T, int op, int64_t B, int dty
     case CTLIB_OP_BPRJ_T_TRUE:
         L.n_ylines = P.ny ? img * es : 0;
         sino_partials(op != CTLIB_OP_PRJ);
+        if (op == CTLIB_OP_PRJ && T.fwd_lt == 0) L.n_ltab = fwd_line_table_bytes(P, (int)B);
         break;
     case CTLIB_OP_PRJ_T:
     case CTLIB_OP_BPRJ:
@@ -246,16 +261,14 @@ WsLayout layout(const ctlib_plan* p, const Tuning& T, int op, int64_t B, int dty
     L.xp = off;     off += align_up(L.n_xp);
     L.sp = off;     off += align_up(L.n_sp);
     L.rows = off;   off += align_up(L.n_rows);
+    L.ltab = off;   off += align_up(L.n_ltab);
     L.total = off;
     return L;
 }
 
-// Views per block the line -> bin launch will use: two views per wave (8 per block) when the
-// edge slots fit (<= 12 per view) unless tuned to 4, else one (4 per block); weighted: 4.
-FwdShape default_fwd_shape(const DevPlan& P, bool weighted, int tuned = 0) {
-    if (weighted) return FwdShape{4, 1};
-    const bool two = (P.D + 1 + 63) / 64 <= 12;
-    return FwdShape{two && tuned != 4 ? 8 : 4, 1};
+// Launch shape of the line -> bin kernel under layout L.
+FwdShape fwd_shape(const DevPlan& P, bool weighted, const WsLayout& L) {
+    return FwdShape{fwd_views(P, weighted, L.fwd_G, !weighted && L.n_ltab > 0), L.fnch};
 }
 
 // ---- kernel timing (HIP event pairs around the main launches, on their stream)
@@ -338,6 +351,12 @@ int check_ws(size_t need, const void* ws, size_t ws_bytes, const char* what) {
     return CTLIB_OK;
 }
 
+hipError_t run_line_tables(const DevPlan& P, int dtype, const void* srcx, const void* srcy, double2* lt, int B,
+                           hipStream_t s) {
+    KernelTimer t(K_AUX, 0, 0, s);
+    return launch_line_tables(P, dtype, srcx, srcy, lt, B, s);
+}
+
 // Sinogram-output ops: projection and backprojection_t.
 hipError_t run_to_sino(const ctlib_plan* p, int dtype, const void* in, void* out, int B, bool weighted,
                        const double* post, const WsLayout& L, char* ws, hipStream_t s) {
@@ -350,12 +369,13 @@ hipError_t run_to_sino(const ctlib_plan* p, int dtype, const void* in, void* out
         e = launch_to_ylines(in, yl, dtype, B, P.H, P.W, s);
         if (e != hipSuccess) return e;
     }
-    FwdShape sh = default_fwd_shape(P, weighted, L.fwd_G);
-    sh.nch = L.fnch;
     void* dst = L.fnch > 1 ? ws + L.sp : out;
+    double2* lt = weighted ? nullptr : L.line_tables(ws);
+    if (lt) e = run_line_tables(P, dtype, in, yl, lt, B, s);
+    if (e != hipSuccess) return e;
     {
         KernelTimer t(K_FWD, sweep_bytes(P, B, dtype), 0, s);
-        e = launch_fwd(P, dtype, in, yl, dst, B, weighted, post, sh, s);
+        e = launch_fwd(P, dtype, in, yl, dst, B, weighted, post, fwd_shape(P, weighted, L), s, lt);
     }
     if (e != hipSuccess || L.fnch <= 1) return e;
     KernelTimer t(K_AUX, 0, 0, s);
@@ -622,6 +642,13 @@ int ctlib_plan_set_tuning(ctlib_plan* p, int fwd_views_per_block) {
     return CTLIB_OK;
 }
 
+int ctlib_plan_set_line_tables(ctlib_plan* p, int per_block) {
+    if (!p) return fail(CTLIB_ERR_INVALID_ARG, "NULL plan");
+    if (per_block != 0 && per_block != 1) return fail(CTLIB_ERR_INVALID_ARG, "per_block must be 0 or 1");
+    p->fwd_lt = per_block;
+    return CTLIB_OK;
+}
+
 int ctlib_plan_set_chunks(ctlib_plan* p, int line_chunks, int view_chunks) {
     if (!p) return fail(CTLIB_ERR_INVALID_ARG, "NULL plan");
     if (line_chunks < 0 || line_chunks > 64 || view_chunks < 0 || view_chunks > 64)
@@ -710,9 +737,14 @@ int ctlib_run_noalloc(const ctlib_plan* p, int op, int dtype, const void* in, vo
     int rc = check_run_args(p, op, dtype, in, out, batch);
     if (rc) return rc;
     if (batch == 0) return CTLIB_OK;
-    // the automatic launch split when its partials fit the scratch array, else fewer chunks
+    // the automatic launch split and prebuilt line tables when they fit the scratch array, else
+    // tables built per block, then fewer chunks
     Tuning T = snap(p);
     WsLayout L = layout(p, T, op, batch, dtype);
+    if (L.total > kScratchBytes && L.n_ltab > 0) {
+        T.fwd_lt = 1;
+        L = layout(p, T, op, batch, dtype);
+    }
     while (L.total > kScratchBytes && (L.fnch > 1 || L.nch > 1)) {
         T.fwd_nch = L.fnch > 1 ? L.fnch / 2 : 1;
         T.adj_nch = L.nch > 1 ? L.nch / 2 : 1;
@@ -840,16 +872,17 @@ int ctlib_run_residual_loss(const ctlib_plan* p, int dtype, const void* x, const
         KernelTimer t(K_AUX, 0, 0, s);
         e = launch_to_ylines(x, yl, dt, B, P.H, P.W, s);
     }
+    double2* lt = la.line_tables(w);
+    if (e == hipSuccess && lt) e = run_line_tables(P, dt, x, P.ny ? yl : nullptr, lt, B, s);
     if (e == hipSuccess && la.fnch > 1) {
         // Batch too small to fill the chip unsplit: the forward is split into line chunks, and
         // one pass sums the partial sinograms (FP64, chunk order), writes y if wanted and forms the
         // residual tables and row sums.  (Forming them instead in the last chunk of every work
         // item, on completion counters, measured 0.17 ms slower at fan-ED 512^2 B=32.)
-        FwdShape sh = default_fwd_shape(P, false, la.fwd_G);
-        sh.nch = la.fnch;
         {
             KernelTimer t(K_FWD, sweep_bytes(P, B, dt), 0, s);
-            e = launch_fwd(P, dt, x, P.ny ? yl : nullptr, w + la.sp, B, false, P.post_prj, sh, s);
+            e = launch_fwd(P, dt, x, P.ny ? yl : nullptr, w + la.sp, B, false, P.post_prj, fwd_shape(P, false, la), s,
+                           lt);
         }
         if (e == hipSuccess) {
             KernelTimer t(K_AUX, 0, 0, s);
@@ -859,8 +892,7 @@ int ctlib_run_residual_loss(const ctlib_plan* p, int dtype, const void* x, const
     } else if (e == hipSuccess) {
         // forward fused with the residual tables (no separate table pass)
         KernelTimer t(K_FWD, sweep_bytes(P, B, dt), 0, s);
-        e = launch_fwd_resid(P, dt, x, P.ny ? yl : nullptr, y, m, tabs, B, default_fwd_shape(P, false, la.fwd_G), s,
-                             row_sq);
+        e = launch_fwd_resid(P, dt, x, P.ny ? yl : nullptr, y, m, tabs, B, fwd_shape(P, false, la), s, row_sq, lt);
     }
     if (e == hipSuccess) e = run_to_image(p, dt, tabs, grad, B, false, 1.0, lb, w, s);
     if (e != hipSuccess) return hip_fail(e, "kernel launch");

@@ -328,6 +328,37 @@ __host__ __device__ inline size_t fwd_block_lds(int nmax, int G, bool weighted, 
     return (size_t)2 * ntab * (nmax + 2) * entry_bytes + (size_t)2 * G * LC_WORDS * 8 + (size_t)2 * ntab * 4 * 8;
 }
 
+// ---- Line tables built once per (slice, line), then streamed into LDS ----
+// Per line TABN = 256*M + 2 entries {C_j, S_j} (16 bytes, j = 0..n+1 used), the same
+// layout in global memory and in LDS, so one linear copy moves a table.
+template <int M>
+__host__ __device__ constexpr int ltab_n() { return NT * M + 2; }
+
+// One line's table by one wave: lane L owns pixels 4M*L .. 4M*L + 4M - 1 (n <= 256*M);
+// the entries are those line_tables writes for the same line.
+template <typename T, int M>
+__device__ __forceinline__ void line_table_wave(const T* __restrict__ row, int n, double2* __restrict__ out, int lane) {
+    constexpr int PPL = 4 * M;
+    const int p0 = lane * PPL;
+    double v[PPL];
+    double run = 0.0;
+#pragma unroll
+    for (int i = 0; i < PPL; i++) {
+        v[i] = p0 + i < n ? (double)row[p0 + i] : 0.0;
+        run += v[i];
+    }
+    const double inc = wave_scan_incl(run, lane);
+    double pref = inc - run;
+#pragma unroll
+    for (int i = 0; i < PPL; i++) {
+        const int p = p0 + i;
+        if (p < n) out[p + 1] = make_double2(pref - (double)(p + 1) * v[i], v[i]);
+        pref += v[i];
+    }
+    if (lane == 0) out[0] = make_double2(0.0, 0.0);
+    if (lane == 63) out[n + 1] = make_double2(inc, 0.0);
+}
+
 // One line -> bin work item: batch element b, sweep axis `mode`, views
 // vlist[grp*G .. grp*G+G-1] of that axis (G = 4*VPW).  `src` holds the lines
 // of that axis.  Wave w owns views w*VPW .. w*VPW+VPW-1 and, for each, the bin
@@ -344,12 +375,18 @@ __host__ __device__ inline size_t fwd_block_lds(int nmax, int G, bool weighted, 
 // evaluation per line for all the wave's views instead of a mostly idle group.
 // Only lines [lb, le) of the axis contribute (le < 0: all): a launch split into
 // line chunks writes each chunk's partial sinogram and sums them afterwards.
-template <typename T, int VPW, int NEM, int M, bool WEIGHTED, bool RESID, bool FULL = false, bool EDGEX = false>
+// STREAM (unweighted): the tables were built beforehand (k_line_tables, `lt` =
+// the axis's line 0, ltab_n<M>() entries per line) and each is copied global ->
+// LDS one line ahead with global_load_lds_dwordx4 (no register staging), so a
+// line costs one barrier and no scan; `src` is not read.
+template <typename T, int VPW, int NEM, int M, bool WEIGHTED, bool RESID, bool FULL = false, bool EDGEX = false,
+          bool STREAM = false>
 __device__ __forceinline__ void fwd_block(const DevPlan& P, const T* __restrict__ src, T* __restrict__ y, int b,
                                           int mode, int grp, const double* __restrict__ post,
                                           const T* __restrict__ m, double2* __restrict__ tabs,
                                           unsigned char* __restrict__ smem, int lb = 0, int le = -1,
-                                          double* __restrict__ rowsq = nullptr) {
+                                          double* __restrict__ rowsq = nullptr, const double2* __restrict__ lt = nullptr) {
+    static_assert(!STREAM || !WEIGHTED, "prebuilt line tables: unweighted only");
     constexpr int G = 4 * VPW;
     constexpr int NTAB = WEIGHTED ? G : 1;
     const int tid = threadIdx.x, lane = tid & 63;
@@ -386,7 +423,7 @@ __device__ __forceinline__ void fwd_block(const DevPlan& P, const T* __restrict_
     const double ex = EDGEX ? P.edges[(size_t)vx * (D + 1) + D] : 0.0;
     double Sx = 0.0;
 
-    const int tabn = n + 2;
+    const int tabn = STREAM ? ltab_n<M>() : n + 2;
     LEnt<T>* tab = reinterpret_cast<LEnt<T>*>(smem);                // [2][NTAB][tabn]
     double* cf = reinterpret_cast<double*>(tab + 2 * NTAB * tabn);  // [2][G][LC_WORDS]
     double* wsum = cf + 2 * G * LC_WORDS;                           // [2][NTAB][4]
@@ -406,16 +443,42 @@ __device__ __forceinline__ void fwd_block(const DevPlan& P, const T* __restrict_
     const bool any = lb < lend;
     double cnext = cthread && any ? crow[(size_t)lb * LC_WORDS] : 0.0;
 
+    // STREAM: line l's table into buffer l & 1; each lane moves 16 bytes to the wave's LDS
+    // base + 16 * lane
+    auto copy_table = [&](int l) {
+        const double2* srct = lt + (size_t)l * tabn;
+        LEnt<T>* dst = tab + (l & 1) * tabn;
+#pragma unroll
+        for (int r = 0; r < (ltab_n<M>() + NT - 1) / NT; r++) {
+            const int j0 = wave * 64 + r * NT;
+            if (j0 + lane < tabn)
+                __builtin_amdgcn_global_load_lds((const void*)(srct + j0 + lane),
+                                                 (__attribute__((address_space(3))) void*)(dst + j0), 16, 0, 0);
+        }
+    };
     T xv[M];
     const T* lb0 = base + (size_t)lb * n;
+    if constexpr (STREAM) {
+        if (any) copy_table(lb);
+    } else {
 #pragma unroll
-    for (int i = 0; i < M; i++) xv[i] = any && (FULL || (i < m_px && p0 + i < n)) ? lb0[p0 + i] : T(0);
+        for (int i = 0; i < M; i++) xv[i] = any && (FULL || (i < m_px && p0 + i < n)) ? lb0[p0 + i] : T(0);
+    }
 
     for (int l = lb; l < lend; l++) {
         LEnt<T>* tb = tab + (l & 1) * NTAB * tabn;
         double* ws = wsum + (l & 1) * NTAB * 4;
         double* cfl = cf + (l & 1) * G * LC_WORDS;
         if (cthread) cfl[tid] = cnext;
+        if constexpr (STREAM) {
+            // line l's table (copied during line l-1) is complete; line l-1's buffer is free
+            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+            __syncthreads();
+            if (l + 1 < lend) {
+                copy_table(l + 1);
+                if (cthread) cnext = crow[(size_t)(l + 1) * LC_WORDS];
+            }
+        }
         if (WEIGHTED) __syncthreads();  // the weights below read cfl
         double val[NTAB][M];
 #pragma unroll
@@ -439,14 +502,14 @@ __device__ __forceinline__ void fwd_block(const DevPlan& P, const T* __restrict_
                 for (int i = 0; i < M; i++) val[t][i] = (FULL || (i < m_px && p0 + i < n)) ? (double)xv[i] : 0.0;
             }
         }
-        line_tables<T, M, NTAB, FULL>(val, m_px, p0, n, tb, tabn, ws, tid, lane, wave);
-        if (l + 1 < lend) {
+        if constexpr (!STREAM) line_tables<T, M, NTAB, FULL>(val, m_px, p0, n, tb, tabn, ws, tid, lane, wave);
+        if (!STREAM && l + 1 < lend) {
             const T* nb = base + (size_t)(l + 1) * n;
 #pragma unroll
             for (int i = 0; i < M; i++) xv[i] = (FULL || (i < m_px && p0 + i < n)) ? nb[p0 + i] : T(0);
             if (cthread) cnext = crow[(size_t)(l + 1) * LC_WORDS];
         }
-        __syncthreads();
+        if constexpr (!STREAM) __syncthreads();
         if (EDGEX) {
             const int g = wave * VPW + vxl;  // per lane
             const double tau = __builtin_fma(cfl[g * LC_WORDS + LC_A], ex, cfl[g * LC_WORDS + LC_B]);

@@ -24,15 +24,53 @@ __global__ __launch_bounds__(256) void k_to_ylines(const T* __restrict__ x, T* _
     ylines_tile<T>(x + (size_t)b * H * W, xT + (size_t)b * H * W, H, W, blockIdx.y * 32, blockIdx.x * 32, tile);
 }
 
-// One launch covers both sweep axes: work items [0, B*ngx) are (b, group of
-// 4*VPW X-axis views) and read image rows from `srcx`; the rest are Y-axis
-// groups reading the bottom-to-top column copy `srcy`.
 // fp32 variants with 24 edge slots per lane and up to 4 pixels per thread
 // (e.g. 1536 bins over 1024-pixel lines) are held to 168 VGPRs: three waves per
 // SIMD instead of two, without spills.
 template <typename T, int NEM, int M, bool WEIGHTED>
 constexpr int fwd_waves() { return sizeof(T) == 4 && !WEIGHTED && NEM > 20 && NEM <= 24 && M <= 4 ? 3 : 1; }
 
+// Line tables of every line of a batch into `lt`: per slice the X-axis lines (rows of
+// srcx, n = W) then the Y-axis lines (rows of srcy, n = H), one wave per line.
+template <typename T, int M>
+__global__ __launch_bounds__(256) void k_line_tables(DevPlan P, const T* __restrict__ srcx, const T* __restrict__ srcy,
+                                                     double2* __restrict__ lt, int B) {
+    const int hx = P.nx ? P.H : 0, wy = P.ny ? P.W : 0;
+    const int nl = hx + wy;
+    const int line = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (line >= B * nl) return;  // whole wave
+    const int b = line / nl, l = line - b * nl;
+    const T* row = l < hx ? srcx + ((size_t)b * P.H + l) * P.W : srcy + ((size_t)b * P.W + (l - hx)) * P.H;
+    line_table_wave<T, M>(row, l < hx ? P.W : P.H, lt + (size_t)line * ltab_n<M>(), threadIdx.x & 63);
+}
+
+// k_fwd over the prebuilt line tables of k_line_tables (unweighted).
+template <typename T, int VPW, int NEM, int M, bool RESID, bool EDGEX>
+__global__ __launch_bounds__(256, (fwd_waves<T, NEM, M, false>())) void k_fwd_lt(
+    DevPlan P, const double2* __restrict__ lt, T* __restrict__ y, int B, int ngx, int ngy,
+    const double* __restrict__ post, const T* __restrict__ m, double2* __restrict__ tabs, int nch,
+    double* __restrict__ rowsq) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int nwx = B * ngx;
+    int w = xcd_remap(blockIdx.x, gridDim.x, (nwx + B * ngy) * nch);
+    if (w < 0) return;
+    const int ch = w % nch;
+    w /= nch;
+    const int mode = w < nwx ? MODE_X : MODE_Y;
+    if (mode == MODE_Y) w -= nwx;
+    const int ngroups = mode == MODE_Y ? ngy : ngx;
+    const int b = w / ngroups, grp = w - b * ngroups;
+    const int nl = mode == MODE_Y ? P.W : P.H;
+    const int hx = P.nx ? P.H : 0, wy = P.ny ? P.W : 0;
+    const double2* lts = lt + ((size_t)b * (hx + wy) + (mode == MODE_Y ? hx : 0)) * ltab_n<M>();
+    fwd_block<T, VPW, NEM, M, false, RESID, false, EDGEX, true>(
+        P, nullptr, y + (size_t)ch * B * P.V * P.D, b, mode, grp, post, m, tabs, smem, ch * nl / nch,
+        (ch + 1) * nl / nch, rowsq, lts);
+}
+
+// One launch covers both sweep axes: work items [0, B*ngx) are (b, group of
+// 4*VPW X-axis views) and read image rows from `srcx`; the rest are Y-axis
+// groups reading the bottom-to-top column copy `srcy`.
 template <typename T, int VPW, int NEM, int M, bool WEIGHTED, bool RESID, bool FULL, bool EDGEX>
 __global__ __launch_bounds__(256, (fwd_waves<T, NEM, M, WEIGHTED>())) void k_fwd(DevPlan P, const T* __restrict__ srcx, const T* __restrict__ srcy,
                                              T* __restrict__ y, int B, int ngx, int ngy,
@@ -151,6 +189,21 @@ hipError_t launch_to_ylines(const void* x, void* xT, int dtype, int B, int H, in
     return hipGetLastError();
 }
 
+int fwd_px(const DevPlan& P) {
+    int n = P.nx ? P.W : 0;
+    if (P.ny && P.H > n) n = P.H;
+    const int m = (n + NT - 1) / NT;
+    if (m <= 1 && (P.nx == 0 || P.W == NT) && (P.ny == 0 || P.H == NT)) return 1;
+    return m <= 2 ? 2 : m <= 4 ? 4 : m <= 8 ? 8 : 0;
+}
+
+size_t fwd_line_table_bytes(const DevPlan& P, int B) {
+    const int M = fwd_px(P);
+    if (B <= 0 || M == 0 || P.nx + P.ny == 0) return 0;
+    const size_t nlt = (size_t)(P.nx ? P.H : 0) + (P.ny ? P.W : 0);
+    return (size_t)B * nlt * (NT * M + 2) * sizeof(double2);
+}
+
 size_t fwd_lds_bytes(const DevPlan& P, bool weighted, int G) {
     const int n = P.nx ? P.W : P.H;
     const int nmax = P.ny && P.H > n ? P.H : n;
@@ -169,6 +222,7 @@ struct FwdArgs {
     double2* tabs;
     int nch;
     double* rowsq;
+    const double2* lt;  // line tables of launch_line_tables (unweighted), or null: each block builds its own
 };
 
 template <typename T, int VPW, int NEM, int M, bool WTD, bool RESID, bool FULL, bool EDGEX>
@@ -178,6 +232,19 @@ hipError_t fwd_launch_t(const DevPlan& P, const FwdArgs& a, hipStream_t s) {
     const long long nw = (long long)(ngx + ngy) * a.B * a.nch;
     if (nw > (1LL << 30)) return hipErrorInvalidValue;  // grid size (one block per work item)
     const int nwork = (int)nw;
+    if constexpr (!WTD) {
+        if (a.lt) {
+            const size_t lds = fwd_block_lds(NT * M, G, false, 16);
+            auto kern = k_fwd_lt<T, VPW, NEM, M, RESID, EDGEX>;
+            if (lds > 64 * 1024) {
+                hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+                if (e != hipSuccess) return e;
+            }
+            hipLaunchKernelGGL(kern, dim3(grid_for(nwork)), dim3(NT), lds, s, P, (const double2*)a.lt, (T*)a.y, a.B,
+                               ngx, ngy, a.post, (const T*)a.m, a.tabs, a.nch, a.rowsq);
+            return hipGetLastError();
+        }
+    }
     const size_t lds = fwd_lds_bytes(P, WTD, G);
     auto kern = k_fwd<T, VPW, NEM, M, WTD, RESID, FULL, EDGEX>;
     if (lds > 64 * 1024) {
@@ -189,21 +256,20 @@ hipError_t fwd_launch_t(const DevPlan& P, const FwdArgs& a, hipStream_t s) {
     return hipGetLastError();
 }
 
-// Pixels per thread of the line table (M >= n / 256).
+// Every swept line (rows of X views, columns of Y views) exactly NT * M pixels.
+bool fwd_full(const DevPlan& P, int M) { return (P.nx == 0 || P.W == NT * M) && (P.ny == 0 || P.H == NT * M); }
+
 template <typename T, int VPW, int NEM, bool WTD, bool RESID, bool EDGEX = false>
 hipError_t fwd_m(const DevPlan& P, const FwdArgs& a, hipStream_t s) {
-    int n = P.nx ? P.W : 0;
-    if (P.ny && P.H > n) n = P.H;
-    const int m = (n + NT - 1) / NT;
-    // every swept line (rows of X views, columns of Y views) exactly NT * M pixels
-    auto full = [&](int M) { return (P.nx == 0 || P.W == NT * M) && (P.ny == 0 || P.H == NT * M); };
-    if (m <= 1 && full(1)) return fwd_launch_t<T, VPW, NEM, 1, WTD, RESID, true, EDGEX>(P, a, s);
-    if (m <= 2) return full(2) ? fwd_launch_t<T, VPW, NEM, 2, WTD, RESID, true, EDGEX>(P, a, s)
-                               : fwd_launch_t<T, VPW, NEM, 2, WTD, RESID, false, EDGEX>(P, a, s);
-    if (m <= 4) return full(4) ? fwd_launch_t<T, VPW, NEM, 4, WTD, RESID, true, EDGEX>(P, a, s)
-                               : fwd_launch_t<T, VPW, NEM, 4, WTD, RESID, false, EDGEX>(P, a, s);
-    if (m <= 8) return fwd_launch_t<T, VPW, NEM, 8, WTD, RESID, false, EDGEX>(P, a, s);
-    return hipErrorInvalidValue;
+    switch (fwd_px(P)) {
+    case 1: return fwd_launch_t<T, VPW, NEM, 1, WTD, RESID, true, EDGEX>(P, a, s);
+    case 2: return fwd_full(P, 2) ? fwd_launch_t<T, VPW, NEM, 2, WTD, RESID, true, EDGEX>(P, a, s)
+                                  : fwd_launch_t<T, VPW, NEM, 2, WTD, RESID, false, EDGEX>(P, a, s);
+    case 4: return fwd_full(P, 4) ? fwd_launch_t<T, VPW, NEM, 4, WTD, RESID, true, EDGEX>(P, a, s)
+                                  : fwd_launch_t<T, VPW, NEM, 4, WTD, RESID, false, EDGEX>(P, a, s);
+    case 8: return fwd_launch_t<T, VPW, NEM, 8, WTD, RESID, false, EDGEX>(P, a, s);
+    default: return hipErrorInvalidValue;
+    }
 }
 
 // Edge slots of 64 per wave and view: NEM >= ceil((D+1)/64); two views per wave only up to NEM 12.
@@ -236,25 +302,47 @@ hipError_t fwd_dispatch(const DevPlan& P, const FwdArgs& a, bool weighted, bool 
     return two ? fwd_ne<T, 2, false, false>(P, a, s) : fwd_ne<T, 1, false, false>(P, a, s);
 }
 
+template <typename T>
+hipError_t line_tables_t(const DevPlan& P, const void* srcx, const void* srcy, double2* lt, int B, hipStream_t s) {
+    const long long ntab = (long long)B * ((P.nx ? P.H : 0) + (P.ny ? P.W : 0));
+    if (ntab > (1LL << 31) - 4) return hipErrorInvalidValue;
+    const dim3 grid((unsigned)((ntab + 3) / 4));
+    switch (fwd_px(P)) {
+    case 1: hipLaunchKernelGGL((k_line_tables<T, 1>), grid, dim3(256), 0, s, P, (const T*)srcx, (const T*)srcy, lt, B); break;
+    case 2: hipLaunchKernelGGL((k_line_tables<T, 2>), grid, dim3(256), 0, s, P, (const T*)srcx, (const T*)srcy, lt, B); break;
+    case 4: hipLaunchKernelGGL((k_line_tables<T, 4>), grid, dim3(256), 0, s, P, (const T*)srcx, (const T*)srcy, lt, B); break;
+    case 8: hipLaunchKernelGGL((k_line_tables<T, 8>), grid, dim3(256), 0, s, P, (const T*)srcx, (const T*)srcy, lt, B); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
 }  // namespace
 
+hipError_t launch_line_tables(const DevPlan& P, int dtype, const void* srcx, const void* srcy, double2* ltab, int B,
+                              hipStream_t s) {
+    if (B <= 0 || P.nx + P.ny == 0) return hipSuccess;
+    if (dtype == DT_F32) return line_tables_t<float>(P, srcx, srcy, ltab, B, s);
+    return line_tables_t<double>(P, srcx, srcy, ltab, B, s);
+}
+
 hipError_t launch_fwd(const DevPlan& P, int dtype, const void* srcx, const void* srcy, void* y, int B, bool weighted,
-                      const double* post, FwdShape shape, hipStream_t s) {
+                      const double* post, FwdShape shape, hipStream_t s, const double2* ltab) {
     if (B <= 0 || P.nx + P.ny == 0) return hipSuccess;
     if ((P.nx && P.W > NT * 8) || (P.ny && P.H > NT * 8) || P.D > 2046) return hipErrorInvalidValue;
     const int nch = shape.nch < 1 ? 1 : shape.nch;
     if ((P.nx && nch > P.H) || (P.ny && nch > P.W)) return hipErrorInvalidValue;
     if (weighted && fwd_w_supported(P)) return launch_fwd_w(P, dtype, srcx, srcy, y, B, post, nch, s);
-    FwdArgs a{srcx, srcy, y, B, post, nullptr, nullptr, nch, nullptr};
+    FwdArgs a{srcx, srcy, y, B, post, nullptr, nullptr, nch, nullptr, weighted ? nullptr : ltab};
     if (dtype == DT_F32) return fwd_dispatch<float>(P, a, weighted, false, shape.G, s);
     return fwd_dispatch<double>(P, a, weighted, false, shape.G, s);
 }
 
 hipError_t launch_fwd_resid(const DevPlan& P, int dtype, const void* srcx, const void* srcy, void* y, const void* m,
-                            double2* tabs, int B, FwdShape shape, hipStream_t s, double* rowsq) {
+                            double2* tabs, int B, FwdShape shape, hipStream_t s, double* rowsq, const double2* ltab) {
     if (B <= 0 || P.nx + P.ny == 0) return hipSuccess;
     if ((P.nx && P.W > NT * 8) || (P.ny && P.H > NT * 8) || P.D > 2046) return hipErrorInvalidValue;
-    FwdArgs a{srcx, srcy, y, B, P.post_prj, m, tabs, 1, rowsq};
+    FwdArgs a{srcx, srcy, y, B, P.post_prj, m, tabs, 1, rowsq, ltab};
     if (dtype == DT_F32) return fwd_dispatch<float>(P, a, false, true, shape.G, s);
     return fwd_dispatch<double>(P, a, false, true, shape.G, s);
 }

@@ -48,18 +48,32 @@ hipError_t launch_to_ylines(const void* x, void* xT, int dtype, int B, int H, in
 // `srcx` holds image rows ([B][H][W]), `srcy` the bottom-to-top column copy ([B][W][H]).
 // With shape.nch = k > 1 each axis's image lines are split into k chunks (k <= lines of each
 // axis) and chunk c writes its partial sinogram at y + c*B*V*D (elements); launch_sum_parts
-// adds them up.
+// adds them up.  ltab (unweighted only; ignored when weighted): the line tables of
+// launch_line_tables for the same srcx / srcy, copied into LDS line by line instead of built
+// per block; null: each block builds the tables of its lines.
 hipError_t launch_fwd(const DevPlan& P, int dtype, const void* srcx, const void* srcy, void* y, int B,
-                      bool weighted, const double* post, FwdShape shape, hipStream_t s);
+                      bool weighted, const double* post, FwdShape shape, hipStream_t s,
+                      const double2* ltab = nullptr);
+// Cumulative table of every image line of both axes (one wave per line) into ltab, of
+// fwd_line_table_bytes(P, B): per slice the X-axis lines (rows of srcx) then the Y-axis lines
+// (rows of srcy), 256*M + 2 entries {C_j, S_j} per line, M = fwd_px(P).
+hipError_t launch_line_tables(const DevPlan& P, int dtype, const void* srcx, const void* srcy, double2* ltab, int B,
+                              hipStream_t s);
 // out[i] = sum_k parts[k*count + i], k < nch (out may equal parts)
 hipError_t launch_sum_parts(void* out, const void* parts, int dtype, int nch, size_t count, hipStream_t s);
 size_t fwd_lds_bytes(const DevPlan& P, bool weighted, int G);
+// Pixels per thread M of the line -> bin launch (lines of at most 256*M pixels; 0: too long).
+int fwd_px(const DevPlan& P);
+// Bytes of the prebuilt line tables of a batch of B: B * (lines of both axes) * (256*M + 2) * 16.
+size_t fwd_line_table_bytes(const DevPlan& P, int B);
 
 // projection fused with the residual bin tables of projection_t: writes y = A x (y may be null)
 // and tabs[b][v] = tables of g = (y - m) * post_prj, so that launch_adj yields A^T (A x - m);
 // rowsq (may be null) receives sum_k (y - m)^2 per row [B][V], y rounded to the data type.
+// ltab as for launch_fwd.
 hipError_t launch_fwd_resid(const DevPlan& P, int dtype, const void* srcx, const void* srcy, void* y, const void* m,
-                            double2* tabs, int B, FwdShape shape, hipStream_t s, double* rowsq = nullptr);
+                            double2* tabs, int B, FwdShape shape, hipStream_t s, double* rowsq = nullptr,
+                            const double2* ltab = nullptr);
 
 // Weighted line -> bin operator (backprojection_t; post may be null) with wave-private line
 // tables, for lines of at most 1024 pixels (fwd_w_supported); launch_fwd dispatches to it.

@@ -309,6 +309,12 @@ void set_plan_chunks(torch::Tensor options, int64_t device, int64_t line_chunks,
     if (rc) raise_status(rc, "set_plan_chunks");
 }
 
+void set_plan_line_tables(torch::Tensor options, int64_t device, int64_t per_block) {
+    Cached c = get_plan(options, (int)device);
+    int rc = ctlib_plan_set_line_tables(c.plan.get(), (int)per_block);
+    if (rc) raise_status(rc, "set_plan_line_tables");
+}
+
 void timing_enable(bool on) { ctlib_timing_enable(on ? 1 : 0); }
 
 // Per kernel class: launches, total milliseconds, algorithmic bytes and flops of the
@@ -461,6 +467,8 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m) {
     m.def("set_plan_tuning", &set_plan_tuning, "Override the line->bin views per block of a cached plan");
     m.def("set_plan_chunks", &set_plan_chunks,
           "Override the line->bin line chunks and bin->pixel view chunks of a cached plan");
+    m.def("set_plan_line_tables", &set_plan_line_tables,
+          "Line->bin line tables of a cached plan: 0 prebuilt in the workspace, 1 built per block");
     m.def("timing_enable", &timing_enable, "Record HIP events around the library's kernel launches");
     m.def("timing_collect", &timing_collect, "Per-class kernel times since the last collect");
     m.attr("OP_PRJ") = (int)CTLIB_OP_PRJ;

@@ -138,6 +138,19 @@ int ctlib_plan_set_tuning(ctlib_plan* plan, int fwd_views_per_block);
  * No reference counterpart (launch shape only). */
 int ctlib_plan_set_chunks(ctlib_plan* plan, int line_chunks, int view_chunks);
 
+/* Where the unweighted line->bin kernel (projection, and the forward of
+ * ctlib_run_residual / ctlib_run_residual_loss) gets the cumulative table of
+ * each image line.  0 (default): one pass builds every line's table into the
+ * workspace first (batch * (H + W) * (256*ceil(max(H, W)/256) + 2) * 16 bytes
+ * at most), and the kernel copies them into LDS; 1: each block builds the
+ * tables of the lines it sweeps (no table workspace, more arithmetic per
+ * block).  Same results up to FP64 summation order.  ctlib_run_noalloc uses 1
+ * by itself when the tables do not fit its scratch.  The workspace size depends
+ * on it: set it before ctlib_workspace_bytes / ctlib_residual_workspace_bytes
+ * and before the plan is used from several threads.  No reference counterpart
+ * (launch shape only). */
+int ctlib_plan_set_line_tables(ctlib_plan* plan, int per_block);
+
 /* Geometry queries for host code that mirrors the plan (tests, autograd). */
 int ctlib_plan_geom(const ctlib_plan* plan, ctlib_geom* out);
 int ctlib_plan_view_info(const ctlib_plan* plan, int view, int* xmode, int* rev);

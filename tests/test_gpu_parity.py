@@ -175,6 +175,40 @@ def test_square_line_lengths(dev, n, st, op):
     assert rel_l2(got, ref) <= F32_TOL, (op, st, n, rel_l2(got, ref))
 
 
+@pytest.mark.parametrize("per_block", [0, 1])
+@pytest.mark.parametrize("wh", [(256, 256), (300, 40), (40, 640), (1024, 64), (1800, 48), (48, 2048)])
+@pytest.mark.parametrize("st", [0, 2])
+def test_forward_line_lengths(dev, st, wh, per_block):
+    """Projection and the residual step over every pixels-per-thread class of the line->bin
+    kernel (lines of 256, 512, 1024 and 2048 pixels at most; whole 256-pixel multiples or not;
+    the two sweep axes of one slice of different lengths), with line tables prebuilt in the
+    workspace (per_block 0) or built by each block (1)."""
+    from ctlib_amd import _C
+    W, H = wh
+    if st == 2:
+        o = phantom.para_options(16, 97, W, H, 2.0 / max(W, H), 0.03, Ang0=0.2, dAng=math.pi / 16, binshift=0.01)
+    else:
+        o = phantom.fan_options(16, 97, W, H, 2.0 / max(W, H), 0.06, 3.0, 2.0, st, Ang0=0.2, binshift=0.01)
+    rng = np.random.default_rng(700 + W + H + st)
+    x = rng.random((2, 1, H, W)).astype(np.float32)
+    mm = rng.random((2, 1, 16, 97)).astype(np.float32)
+    ot = torch.from_numpy(o)
+    print(f"[case] forward line lengths {W}x{H} st={st} per_block={per_block}", flush=True)
+    try:
+        _C.set_plan_line_tables(ot, dev.index or 0, per_block)
+        xt, mt = torch.from_numpy(x).to(dev), torch.from_numpy(mm).to(dev)
+        y = ctlib_amd.projection(xt, ot).cpu().numpy()
+        yr, gr = ctlib_amd.residual_step(xt, mt, ot)
+    finally:
+        _C.set_plan_line_tables(ot, dev.index or 0, 0)
+    o64 = o.astype(np.float64)
+    y_ref = O.apply("projection", o64, x[:, 0].astype(np.float64))
+    g_ref = O.apply("projection_t", o64, y_ref - mm[:, 0].astype(np.float64))
+    assert rel_l2(y[:, 0], y_ref) <= F32_TOL
+    assert rel_l2(yr.cpu().numpy()[:, 0], y_ref) <= F32_TOL
+    assert rel_l2(gr.cpu().numpy()[:, 0], g_ref) <= F32_TOL
+
+
 @pytest.mark.parametrize("D", [512, 1024, 1536])
 @pytest.mark.parametrize("st", [0, 2])
 @pytest.mark.parametrize("op", ["projection", "backprojection_t"])
@@ -281,10 +315,12 @@ def test_residual_step(dev):
     assert rel_l2(grad.cpu().numpy(), g_ref.cpu().numpy()) <= 1e-6
 
 
+@pytest.mark.parametrize("per_block", [0, 1])
 @pytest.mark.parametrize("G", [4, 8])
 @pytest.mark.parametrize("case", ["ed_small", "ea_odd", "para_ties", "config1"])
-def test_forward_launch_shapes(dev, case, G):
-    """Both line->bin launch shapes the autotuner may pick (4 or 8 views per block) give the
+def test_forward_launch_shapes(dev, case, G, per_block):
+    """Both line->bin launch shapes the autotuner may pick (4 or 8 views per block), over line
+    tables prebuilt in the workspace (per_block 0) or built by each block (1), give the
     oracle's projection and residual step."""
     from ctlib_amd import _C
     if case == "config1":
@@ -300,14 +336,16 @@ def test_forward_launch_shapes(dev, case, G):
     x = rng.random((2, 1, g["height"], g["width"])).astype(np.float32)
     mm = rng.random((2, 1, g["views"], g["dets"])).astype(np.float32)
     ot = torch.from_numpy(o)
-    print(f"[case] forward views per block {G} {case}", flush=True)
+    print(f"[case] forward views per block {G} line tables per block {per_block} {case}", flush=True)
     try:
         _C.set_plan_tuning(ot, dev.index or 0, G)
+        _C.set_plan_line_tables(ot, dev.index or 0, per_block)
         xt, mt = torch.from_numpy(x).to(dev), torch.from_numpy(mm).to(dev)
         y = ctlib_amd.projection(xt, ot).cpu().numpy()
         yr, gr = ctlib_amd.residual_step(xt, mt, ot)
     finally:
         _C.set_plan_tuning(ot, dev.index or 0, 0)
+        _C.set_plan_line_tables(ot, dev.index or 0, 0)
     o64 = o.astype(np.float64)
     y_ref = O.apply("projection", o64, x[:, 0].astype(np.float64))
     g_ref = O.apply("projection_t", o64, y_ref - mm[:, 0].astype(np.float64))
