@@ -375,18 +375,12 @@ __device__ __forceinline__ void line_table_wave(const T* __restrict__ row, int n
 // evaluation per line for all the wave's views instead of a mostly idle group.
 // Only lines [lb, le) of the axis contribute (le < 0: all): a launch split into
 // line chunks writes each chunk's partial sinogram and sums them afterwards.
-// STREAM (unweighted): the tables were built beforehand (k_line_tables, `lt` =
-// the axis's line 0, ltab_n<M>() entries per line) and each is copied global ->
-// LDS one line ahead with global_load_lds_dwordx4 (no register staging), so a
-// line costs one barrier and no scan; `src` is not read.
-template <typename T, int VPW, int NEM, int M, bool WEIGHTED, bool RESID, bool FULL = false, bool EDGEX = false,
-          bool STREAM = false>
+template <typename T, int VPW, int NEM, int M, bool WEIGHTED, bool RESID, bool FULL = false, bool EDGEX = false>
 __device__ __forceinline__ void fwd_block(const DevPlan& P, const T* __restrict__ src, T* __restrict__ y, int b,
                                           int mode, int grp, const double* __restrict__ post,
                                           const T* __restrict__ m, double2* __restrict__ tabs,
                                           unsigned char* __restrict__ smem, int lb = 0, int le = -1,
-                                          double* __restrict__ rowsq = nullptr, const double2* __restrict__ lt = nullptr) {
-    static_assert(!STREAM || !WEIGHTED, "prebuilt line tables: unweighted only");
+                                          double* __restrict__ rowsq = nullptr) {
     constexpr int G = 4 * VPW;
     constexpr int NTAB = WEIGHTED ? G : 1;
     const int tid = threadIdx.x, lane = tid & 63;
@@ -423,7 +417,7 @@ __device__ __forceinline__ void fwd_block(const DevPlan& P, const T* __restrict_
     const double ex = EDGEX ? P.edges[(size_t)vx * (D + 1) + D] : 0.0;
     double Sx = 0.0;
 
-    const int tabn = STREAM ? ltab_n<M>() : n + 2;
+    const int tabn = n + 2;
     LEnt<T>* tab = reinterpret_cast<LEnt<T>*>(smem);                // [2][NTAB][tabn]
     double* cf = reinterpret_cast<double*>(tab + 2 * NTAB * tabn);  // [2][G][LC_WORDS]
     double* wsum = cf + 2 * G * LC_WORDS;                           // [2][NTAB][4]
@@ -443,42 +437,16 @@ __device__ __forceinline__ void fwd_block(const DevPlan& P, const T* __restrict_
     const bool any = lb < lend;
     double cnext = cthread && any ? crow[(size_t)lb * LC_WORDS] : 0.0;
 
-    // STREAM: line l's table into buffer l & 1; each lane moves 16 bytes to the wave's LDS
-    // base + 16 * lane
-    auto copy_table = [&](int l) {
-        const double2* srct = lt + (size_t)l * tabn;
-        LEnt<T>* dst = tab + (l & 1) * tabn;
-#pragma unroll
-        for (int r = 0; r < (ltab_n<M>() + NT - 1) / NT; r++) {
-            const int j0 = wave * 64 + r * NT;
-            if (j0 + lane < tabn)
-                __builtin_amdgcn_global_load_lds((const void*)(srct + j0 + lane),
-                                                 (__attribute__((address_space(3))) void*)(dst + j0), 16, 0, 0);
-        }
-    };
     T xv[M];
     const T* lb0 = base + (size_t)lb * n;
-    if constexpr (STREAM) {
-        if (any) copy_table(lb);
-    } else {
 #pragma unroll
-        for (int i = 0; i < M; i++) xv[i] = any && (FULL || (i < m_px && p0 + i < n)) ? lb0[p0 + i] : T(0);
-    }
+    for (int i = 0; i < M; i++) xv[i] = any && (FULL || (i < m_px && p0 + i < n)) ? lb0[p0 + i] : T(0);
 
     for (int l = lb; l < lend; l++) {
         LEnt<T>* tb = tab + (l & 1) * NTAB * tabn;
         double* ws = wsum + (l & 1) * NTAB * 4;
         double* cfl = cf + (l & 1) * G * LC_WORDS;
         if (cthread) cfl[tid] = cnext;
-        if constexpr (STREAM) {
-            // line l's table (copied during line l-1) is complete; line l-1's buffer is free
-            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-            __syncthreads();
-            if (l + 1 < lend) {
-                copy_table(l + 1);
-                if (cthread) cnext = crow[(size_t)(l + 1) * LC_WORDS];
-            }
-        }
         if (WEIGHTED) __syncthreads();  // the weights below read cfl
         double val[NTAB][M];
 #pragma unroll
@@ -502,14 +470,14 @@ __device__ __forceinline__ void fwd_block(const DevPlan& P, const T* __restrict_
                 for (int i = 0; i < M; i++) val[t][i] = (FULL || (i < m_px && p0 + i < n)) ? (double)xv[i] : 0.0;
             }
         }
-        if constexpr (!STREAM) line_tables<T, M, NTAB, FULL>(val, m_px, p0, n, tb, tabn, ws, tid, lane, wave);
-        if (!STREAM && l + 1 < lend) {
+        line_tables<T, M, NTAB, FULL>(val, m_px, p0, n, tb, tabn, ws, tid, lane, wave);
+        if (l + 1 < lend) {
             const T* nb = base + (size_t)(l + 1) * n;
 #pragma unroll
             for (int i = 0; i < M; i++) xv[i] = (FULL || (i < m_px && p0 + i < n)) ? nb[p0 + i] : T(0);
             if (cthread) cnext = crow[(size_t)(l + 1) * LC_WORDS];
         }
-        if constexpr (!STREAM) __syncthreads();
+        __syncthreads();
         if (EDGEX) {
             const int g = wave * VPW + vxl;  // per lane
             const double tau = __builtin_fma(cfl[g * LC_WORDS + LC_A], ex, cfl[g * LC_WORDS + LC_B]);
@@ -534,6 +502,143 @@ __device__ __forceinline__ void fwd_block(const DevPlan& P, const T* __restrict_
                 S[v][i] = __builtin_fma(s, __builtin_fma(tau, sl, c), S[v][i]);
             }
         }
+    }
+
+    fwd_store<T, VPW, NEM, RESID, EDGEX>(P, S, vw, grp, b, mode, lane, wave, y, post, m, tabs, Sx, rowsq);
+}
+
+// LDS bytes of one fwd_block_lt: NB line-table buffers and NB LineCoef buffers.
+template <int M>
+__host__ __device__ inline size_t fwd_lt_lds(int G, int NB) {
+    return (size_t)NB * ltab_n<M>() * 16 + (size_t)NB * G * LC_WORDS * 8;
+}
+
+template <int N>
+__device__ __forceinline__ void wait_vm_lgkm() {
+    asm volatile("s_waitcnt vmcnt(%0) lgkmcnt(0)" ::"n"(N) : "memory");
+}
+
+// fwd_block over prebuilt line tables (unweighted; k_line_tables): `lt` holds the axis's
+// lines, ltab_n<M>() entries each, so a line costs no scan and one barrier, and `src` is not
+// needed.  Line l's table and the G LineCoefs of line l are copied global -> LDS by
+// global_load_lds_dwordx4 (16 bytes per lane straight into LDS, no register staging; the
+// LineCoefs by wave 0's lanes < 4G) NB - 1 lines ahead into buffer l % NB.  NB = 2 is used:
+// three buffers (two lines ahead) measured 2-4 % slower on 1024-pixel lines, where their LDS
+// lowers the blocks per CU.  Per line one wait and one barrier: each wave waits until at most
+// the copies of the lines after l are outstanding (loads complete in order), then the barrier
+// makes every wave's copies of line l visible and frees the buffer of line l - 1.  Wave w
+// issues ceil((TABN - 64 w) / 256) table copies per line (M + 1 for wave 0, M for the
+// others) and wave 0 one LineCoef copy; the waits count exactly those.
+template <typename T, int VPW, int NEM, int M, bool RESID, bool EDGEX, int NB>
+__device__ __forceinline__ void fwd_block_lt(const DevPlan& P, const double2* __restrict__ lt, T* __restrict__ y,
+                                             int b, int mode, int grp, const double* __restrict__ post,
+                                             const T* __restrict__ m, double2* __restrict__ tabs,
+                                             unsigned char* __restrict__ smem, int lb, int le,
+                                             double* __restrict__ rowsq) {
+    static_assert(NB == 2 || NB == 3, "two or three buffers");
+    constexpr int G = 4 * VPW;
+    constexpr int TABN = ltab_n<M>();
+    static_assert(4 * G <= 64, "one LineCoef copy instruction per line");
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int* vlist = mode == MODE_Y ? P.yviews : P.xviews;
+    const int nv = mode == MODE_Y ? P.ny : P.nx;
+    const int nlines = mode == MODE_Y ? P.W : P.H;
+    const int lend = le < 0 ? nlines : le;
+    const int n = mode == MODE_Y ? P.H : P.W;
+    const int D = P.D;
+
+    int vw[VPW];
+#pragma unroll
+    for (int v = 0; v < VPW; v++) {
+        const int idx = grp * G + wave * VPW + v;
+        vw[v] = vlist[idx < nv ? idx : nv - 1];
+    }
+    double e[VPW][NEM], S[VPW][NEM];
+#pragma unroll
+    for (int v = 0; v < VPW; v++) {
+#pragma unroll
+        for (int i = 0; i < NEM; i++) {
+            const int k = lane + 64 * i;
+            e[v][i] = P.edges[(size_t)vw[v] * (D + 1) + (k <= D ? k : D)];
+            S[v][i] = 0.0;
+        }
+    }
+    const int vxl = lane < VPW ? lane : 0;
+    int vx = vw[0];
+#pragma unroll
+    for (int v = 1; v < VPW; v++)
+        if (vxl == v) vx = vw[v];
+    const double ex = EDGEX ? P.edges[(size_t)vx * (D + 1) + D] : 0.0;
+    double Sx = 0.0;
+
+    LEnt<T>* tab = reinterpret_cast<LEnt<T>*>(smem);             // [NB][TABN]
+    double* cf = reinterpret_cast<double*>(tab + NB * TABN);    // [NB][G][LC_WORDS]
+    const int hi = n + 1;
+    const bool clane = wave == 0 && lane < 4 * G;
+    const char* csrc = nullptr;
+    if (clane) {
+        const int idx = grp * G + (lane >> 2);
+        csrc = reinterpret_cast<const char*>(P.lines + (size_t)vlist[idx < nv ? idx : nv - 1] * P.max_lines) +
+               16 * (lane & 3);
+    }
+    auto issue = [&](int l, int buf) {
+        const double2* srct = lt + (size_t)l * TABN;
+        LEnt<T>* dst = tab + buf * TABN;
+#pragma unroll
+        for (int r = 0; r < (TABN + NT - 1) / NT; r++) {
+            const int j0 = wave * 64 + r * NT;
+            if (j0 < TABN && j0 + lane < TABN)
+                __builtin_amdgcn_global_load_lds((const void*)(srct + j0 + lane),
+                                                 (__attribute__((address_space(3))) void*)(dst + j0), 16, 0, 0);
+        }
+        if (clane)
+            __builtin_amdgcn_global_load_lds((const void*)(csrc + (size_t)l * sizeof(LineCoef)),
+                                             (__attribute__((address_space(3))) void*)(cf + buf * G * LC_WORDS), 16,
+                                             0, 0);
+    };
+#pragma unroll
+    for (int k = 0; k < NB - 1; k++)
+        if (lb + k < lend) issue(lb + k, k);
+
+    int bcur = 0;
+    for (int l = lb; l < lend; l++) {
+        // copies of line l complete (at most those of line l + 1 outstanding when NB = 3)
+        if (NB == 3 && l + 1 < lend) {
+            if (wave == 0)
+                wait_vm_lgkm<M + 2>();
+            else
+                wait_vm_lgkm<M>();
+        } else {
+            wait_vm_lgkm<0>();
+        }
+        __builtin_amdgcn_s_barrier();
+        asm volatile("" ::: "memory");
+        if (l + NB - 1 < lend) issue(l + NB - 1, bcur == 0 ? NB - 1 : bcur - 1);
+        const LEnt<T>* tb = tab + bcur * TABN;
+        const double* cfl = cf + bcur * G * LC_WORDS;
+        if (EDGEX) {
+            const int g = wave * VPW + vxl;
+            const double tau = __builtin_fma(cfl[g * LC_WORDS + LC_A], ex, cfl[g * LC_WORDS + LC_B]);
+            double c, sl;
+            lent_read(tb + table_index_d(tau, hi), c, sl);
+            Sx = __builtin_fma(cfl[g * LC_WORDS + LC_S], __builtin_fma(tau, sl, c), Sx);
+        }
+#pragma unroll
+        for (int v = 0; v < VPW; v++) {
+            const int g = wave * VPW + v;
+            const double s = cfl[g * LC_WORDS + LC_S];
+            const double A = cfl[g * LC_WORDS + LC_A];
+            const double Bc = cfl[g * LC_WORDS + LC_B];
+#pragma unroll
+            for (int i = 0; i < NEM; i++) {
+                const double tau = __builtin_fma(A, e[v][i], Bc);
+                double c, sl;
+                lent_read(tb + table_index_d(tau, hi), c, sl);
+                S[v][i] = __builtin_fma(s, __builtin_fma(tau, sl, c), S[v][i]);
+            }
+        }
+        bcur = bcur == NB - 1 ? 0 : bcur + 1;
     }
 
     fwd_store<T, VPW, NEM, RESID, EDGEX>(P, S, vw, grp, b, mode, lane, wave, y, post, m, tabs, Sx, rowsq);

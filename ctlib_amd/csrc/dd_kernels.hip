@@ -44,6 +44,9 @@ __global__ __launch_bounds__(256) void k_line_tables(DevPlan P, const T* __restr
     line_table_wave<T, M>(row, l < hx ? P.W : P.H, lt + (size_t)line * ltab_n<M>(), threadIdx.x & 63);
 }
 
+// Line-table buffers of k_fwd_lt (one line ahead).
+constexpr int FWD_LT_BUFFERS = 2;
+
 // k_fwd over the prebuilt line tables of k_line_tables (unweighted).
 template <typename T, int VPW, int NEM, int M, bool RESID, bool EDGEX>
 __global__ __launch_bounds__(256, (fwd_waves<T, NEM, M, false>())) void k_fwd_lt(
@@ -63,9 +66,8 @@ __global__ __launch_bounds__(256, (fwd_waves<T, NEM, M, false>())) void k_fwd_lt
     const int nl = mode == MODE_Y ? P.W : P.H;
     const int hx = P.nx ? P.H : 0, wy = P.ny ? P.W : 0;
     const double2* lts = lt + ((size_t)b * (hx + wy) + (mode == MODE_Y ? hx : 0)) * ltab_n<M>();
-    fwd_block<T, VPW, NEM, M, false, RESID, false, EDGEX, true>(
-        P, nullptr, y + (size_t)ch * B * P.V * P.D, b, mode, grp, post, m, tabs, smem, ch * nl / nch,
-        (ch + 1) * nl / nch, rowsq, lts);
+    fwd_block_lt<T, VPW, NEM, M, RESID, EDGEX, FWD_LT_BUFFERS>(P, lts, y + (size_t)ch * B * P.V * P.D, b, mode, grp, post,
+                                                          m, tabs, smem, ch * nl / nch, (ch + 1) * nl / nch, rowsq);
 }
 
 // One launch covers both sweep axes: work items [0, B*ngx) are (b, group of
@@ -234,7 +236,7 @@ hipError_t fwd_launch_t(const DevPlan& P, const FwdArgs& a, hipStream_t s) {
     const int nwork = (int)nw;
     if constexpr (!WTD) {
         if (a.lt) {
-            const size_t lds = fwd_block_lds(NT * M, G, false, 16);
+            const size_t lds = fwd_lt_lds<M>(G, FWD_LT_BUFFERS);
             auto kern = k_fwd_lt<T, VPW, NEM, M, RESID, EDGEX>;
             if (lds > 64 * 1024) {
                 hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
