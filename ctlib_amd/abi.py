@@ -48,7 +48,8 @@ LINE_DTYPE = np.dtype({"names": ["a", "s", "A", "B", "w0", "w1", "w2", "inv_s"],
 TABLES = {"views": (0, VIEW_DTYPE), "edges": (1, np.float64), "post_prj": (2, np.float64),
           "post_bprj_t": (3, np.float64), "lines": (4, LINE_DTYPE), "xviews": (5, np.int32),
           "yviews": (6, np.int32), "taps": (7, np.float64), "preweight": (8, np.float64),
-          "ea_atan": (9, np.float32), "adj_lk": (10, np.float32)}
+          "ea_atan": (9, np.float32), "adj_lk": (10, np.float32),
+          "adj_wk": (11, np.float32)}
 
 _lib = None
 

@@ -555,6 +555,7 @@ int ctlib_plan_create(const ctlib_geom* geom, int device, ctlib_plan** out) {
     const size_t o_pw = take(sizeof(double) * H.ramp_preweight.size());
     const size_t o_lkd = take(sizeof(double) * H.lookup_d.size());
     const size_t o_alk = take(sizeof(float) * H.adj_lk.size());
+    const size_t o_awk = take(sizeof(float) * H.adj_wk.size());
     const size_t total = off;
 
     DeviceGuard guard(device);
@@ -581,6 +582,7 @@ int ctlib_plan_create(const ctlib_geom* geom, int device, ctlib_plan** out) {
         {o_pw, H.ramp_preweight.data(), sizeof(double) * H.ramp_preweight.size()},
         {o_lkd, H.lookup_d.data(), sizeof(double) * H.lookup_d.size()},
         {o_alk, H.adj_lk.data(), sizeof(float) * H.adj_lk.size()},
+        {o_awk, H.adj_wk.data(), sizeof(float) * H.adj_wk.size()},
     };
     for (const Copy& c : copies) {
         if (!c.bytes) continue;
@@ -604,6 +606,7 @@ int ctlib_plan_create(const ctlib_geom* geom, int device, ctlib_plan** out) {
     P.preweight = reinterpret_cast<const double*>(base + o_pw);
     P.lookup_d = reinterpret_cast<const double*>(base + o_lkd);
     P.adj_lk = reinterpret_cast<const float4*>(base + o_alk);
+    P.adj_wk = reinterpret_cast<const float4*>(base + o_awk);
     P.nx = (int)H.xviews.size();
     P.ny = (int)H.yviews.size();
     P.V = V;
@@ -706,6 +709,7 @@ int ctlib_plan_export(const ctlib_geom* geom, int table, void* dst, size_t* byte
     case 7: src = H.ramp_taps.data(); n = H.ramp_taps.size() * sizeof(double); break;
     case 8: src = H.ramp_preweight.data(); n = H.ramp_preweight.size() * sizeof(double); break;
     case 10: src = H.adj_lk.data(); n = H.adj_lk.size() * sizeof(float); break;
+    case 11: src = H.adj_wk.data(); n = H.adj_wk.size() * sizeof(float); break;
     case 9:
         ea[0] = H.ea_c[0]; ea[1] = H.ea_c[1]; ea[2] = H.ea_c[2]; ea[3] = H.ea_c[3]; ea[4] = H.ea_c[4];
         ea[5] = H.ea_c[5]; ea[6] = H.ea_T; ea[7] = (float)H.ea_poly;

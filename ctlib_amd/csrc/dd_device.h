@@ -884,7 +884,7 @@ __device__ __forceinline__ double eval_G_lk(const double2* __restrict__ t, doubl
 
 // LDS bytes of one adj_block.
 __host__ __device__ inline size_t adj_block_lds(int D, int L) {
-    return (size_t)2 * (D + 2) * 16 + (size_t)2 * L * LC_WORDS * 8 + (size_t)2 * L * 16;
+    return (size_t)2 * (D + 2) * 16 + (size_t)2 * L * LC_WORDS * 8 + (size_t)4 * L * 16;
 }
 
 // One bin -> pixel work item: batch element b, sweep axis `mode`, image lines
@@ -920,7 +920,9 @@ __device__ __forceinline__ void adj_block(const DevPlan& P, const double2* __res
     double2* stab = reinterpret_cast<double2*>(smem);          // [2][tabn]
     double* scf = reinterpret_cast<double*>(stab + 2 * tabn);  // [2][L][LC_WORDS]
     float4* slk = reinterpret_cast<float4*>(scf + 2 * L * LC_WORDS);  // [2][L] composed lookups (fp32 data)
+    float4* swk = slk + 2 * L;  // [2][L] pre-scaled pixel-weight coefficients (fp32 data, weighted)
     constexpr bool LK = sizeof(T) == 4;
+    constexpr bool WK = WEIGHTED && LK;
 
     const double2* tb_b = tabs + (size_t)b * P.V * tabn;
     // Asynchronous copy of view v's table into LDS buffer `buf`: wave w copies
@@ -947,6 +949,9 @@ __device__ __forceinline__ void adj_block(const DevPlan& P, const double2* __res
     auto lk_word = [&](int v) {
         return athread && al < nlines ? P.adj_lk[(size_t)v * P.max_lines + al] : make_float4(0.f, 0.f, 0.f, 1.f);
     };
+    auto wk_word = [&](int v) {
+        return athread && al < nlines ? P.adj_wk[(size_t)v * P.max_lines + al] : make_float4(0.f, 0.f, 0.f, 1.f);
+    };
 
     // Tiles of 16 or more lines (DMA): the LineCoefs (16 bytes of line tid/4 per thread
     // tid < 4L) and composed lookups (thread tid < L) are copied global -> LDS like the table,
@@ -965,6 +970,9 @@ __device__ __forceinline__ void adj_block(const DevPlan& P, const double2* __res
         if (LK && tid < L)
             __builtin_amdgcn_global_load_lds((const void*)(P.adj_lk + (size_t)v * P.max_lines + min(al, nlines - 1)),
                                              (__attribute__((address_space(3))) void*)(slk + buf * L), 16, 0, 0);
+        if (WK && tid < L)
+            __builtin_amdgcn_global_load_lds((const void*)(P.adj_wk + (size_t)v * P.max_lines + min(al, nlines - 1)),
+                                             (__attribute__((address_space(3))) void*)(swk + buf * L), 16, 0, 0);
     };
     auto view_barrier = [&]() {
         if (DMA) {
@@ -991,6 +999,7 @@ __device__ __forceinline__ void adj_block(const DevPlan& P, const double2* __res
     } else {
         if (cthread) scf[tid] = coef_word(vlist[vb]);
         if (athread) slk[tid] = lk_word(vlist[vb]);
+        if (WK && athread) swk[tid] = wk_word(vlist[vb]);
     }
     view_barrier();
 
@@ -998,7 +1007,7 @@ __device__ __forceinline__ void adj_block(const DevPlan& P, const double2* __res
         const int cur = (iv - vb) & 1;
         const int v = vlist[iv];
         double cnext = 0.0;
-        float4 anext = make_float4(0.f, 0.f, 0.f, 1.f);
+        float4 anext = make_float4(0.f, 0.f, 0.f, 1.f), wnext = anext;
         if (iv + 1 < nv) {
             const int vn = vlist[iv + 1];
             table_to_lds(vn, cur ^ 1);
@@ -1007,11 +1016,13 @@ __device__ __forceinline__ void adj_block(const DevPlan& P, const double2* __res
             } else {
                 cnext = coef_word(vn);
                 if (athread) anext = lk_word(vn);
+                if (WK && athread) wnext = wk_word(vn);
             }
         }
         const double2* t = stab + cur * tabn;
         const double* cf = scf + cur * L * LC_WORDS;
         const float4* clk = slk + cur * L;
+        const float4* cwk = swk + cur * L;
         const Lookup lu = view_lookup(P, v);
         double gx = 0.0;  // EXACT: G at edge n of line lx (lanes >= LPW: unused)
         if (EXACT) {
@@ -1043,21 +1054,26 @@ __device__ __forceinline__ void adj_block(const DevPlan& P, const double2* __res
 #pragma unroll
                 for (int i = 0; i < NG; i++) acc[q][i] += G_at(i);
             } else {
+                // Pixel weight inv_s / ((w0 + w1 p)^2 + w2) of pixel p = lane + 64 k (the call's
+                // scale multiplies the output).  fp32 storage: from the plan's pre-scaled fp32
+                // coefficients, 1 / (u^2 + c^2 w2) with u = k (64 c w1) + u0 (relative error
+                // ~3e-7, below the output's rounding); fp64 storage: u and d in fp64,
+                // Newton-refined reciprocal.
                 const double w0 = cf[lq * LC_WORDS + LC_W0], w1 = cf[lq * LC_WORDS + LC_W1];
                 const double w2 = cf[lq * LC_WORDS + LC_W2];
-                const double ws = cf[lq * LC_WORDS + LC_INVS] * wscale;
-                // Pixel weight ws / ((w0 + w1 p)^2 + w2).  fp32 storage: evaluated in fp32
-                // (relative error ~2e-7, below the output's rounding); fp64 storage: u and
-                // d in fp64, Newton-refined reciprocal.
-                const float w0f = (float)w0, w1f = (float)w1, w2f = (float)w2, wsf = (float)ws;
-                auto wgt = [&](int p) {
-                    if (sizeof(T) == 4) {
-                        const float u = __builtin_fmaf(w1f, (float)p, w0f);
-                        return (double)(wsf * __builtin_amdgcn_rcpf(__builtin_fmaf(u, u, w2f)));
+                const double ws = cf[lq * LC_WORDS + LC_INVS];
+                float4 wk = make_float4(0.f, 0.f, 0.f, 1.f);
+                if (WK) wk = cwk[lq];
+                const float u0 = __builtin_fmaf(wk.y, (float)lane, wk.z);
+                auto wgt = [&](int k) {
+                    if constexpr (WK) {
+                        const float u = k == 0 ? u0 : __builtin_fmaf((float)k, wk.x, u0);
+                        return (double)__builtin_amdgcn_rcpf(__builtin_fmaf(u, u, wk.w));
+                    } else {
+                        const double u = __builtin_fma(w1, (double)(lane + 64 * k), w0);
+                        const double d = __builtin_fma(u, u, w2);
+                        return ws * rcp_refined(d);
                     }
-                    const double u = __builtin_fma(w1, (double)p, w0);
-                    const double d = __builtin_fma(u, u, w2);
-                    return ws * rcp_refined(d);
                 };
                 // right edge of pixel lane + 64 (i-1): edge lane+1 of group i-1, or for
                 // lane 63 edge 0 of group i; both come from the rotations of the groups
@@ -1069,7 +1085,7 @@ __device__ __forceinline__ void adj_block(const DevPlan& P, const double2* __res
                     if (i > 0) {
                         const double right = lane == 63 ? ri : rprev;
                         // pixels past the line accumulate unused values (never stored)
-                        acc[q][i - 1] += wgt(lane + 64 * (i - 1)) * (right - gprev);
+                        acc[q][i - 1] += wgt(i - 1) * (right - gprev);
                     }
                     gprev = gi;
                     rprev = ri;
@@ -1077,15 +1093,16 @@ __device__ __forceinline__ void adj_block(const DevPlan& P, const double2* __res
                 if (EXACT) {
                     // last group: lane 63's pixel n-1 ends at edge n, evaluated by lane q
                     const double gxq = readlane_d(gx, q);
-                    acc[q][NG - 1] += wgt(lane + 64 * (NG - 1)) * ((lane == 63 ? gxq : rprev) - gprev);
+                    acc[q][NG - 1] += wgt(NG - 1) * ((lane == 63 ? gxq : rprev) - gprev);
                 } else {
                     // pixels of the last slot group (lane 63's pixel is past the line since 64 * NPE > n)
-                    acc[q][NPE - 1] += wgt(lane + 64 * (NPE - 1)) * (rprev - gprev);
+                    acc[q][NPE - 1] += wgt(NPE - 1) * (rprev - gprev);
                 }
             }
         }
         if (!DMA && iv + 1 < nv && cthread) scf[(cur ^ 1) * L * LC_WORDS + tid] = cnext;
         if (!DMA && iv + 1 < nv && athread) slk[(cur ^ 1) * L + tid] = anext;
+        if (!DMA && WK && iv + 1 < nv && athread) swk[(cur ^ 1) * L + tid] = wnext;
         view_barrier();
     }
 
@@ -1110,7 +1127,7 @@ __device__ __forceinline__ void adj_block(const DevPlan& P, const double2* __res
                 if (lane == 63) right = r0;
                 val = right - acc[q][i];
             } else {
-                val = acc[q][i];
+                val = acc[q][i] * wscale;
             }
             if (p < n) dst[p] = (T)val;
         }

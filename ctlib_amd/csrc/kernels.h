@@ -22,6 +22,7 @@ struct DevPlan {
     const double* preweight;  // [D]
     const double* lookup_d;   // [V][6] lookup coefficients in fp64 (used for fp64 data)
     const float4* adj_lk;     // [V][max_lines] lookup composed with the line's pixel-edge map (HostPlan)
+    const float4* adj_wk;     // [V][max_lines] pre-scaled pixel-weight coefficients (HostPlan)
     int nx, ny;
     int V, D, W, H, max_lines;
     int scan_type;

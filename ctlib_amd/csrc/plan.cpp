@@ -141,6 +141,7 @@ int build_plan(const Geom& g, HostPlan* P, std::string* why) {
     P->post_bprj_t.assign((size_t)V * D, 1.0);
     P->lines.assign((size_t)V * P->max_lines, LineCoef{});
     P->adj_lk.assign((size_t)V * P->max_lines * 4, 0.0f);
+    P->adj_wk.assign((size_t)V * P->max_lines * 4, 0.0f);
     P->lookup_d.assign((size_t)V * 6, 0.0);
     P->xviews.clear();
     P->yviews.clear();
@@ -309,6 +310,12 @@ int build_plan(const Geom& g, HostPlan* P, std::string* why) {
             ck[1] = (float)(lkd[0] * c.a + lkd[1]);
             ck[2] = (float)(64.0 * lkd[2] * c.s);
             ck[3] = (float)(lkd[2] * c.a + lkd[3]);
+            const double cw = std::sqrt(c.s);
+            float* wk = &P->adj_wk[((size_t)v * P->max_lines + l) * 4];
+            wk[0] = (float)(64.0 * cw * c.w1);
+            wk[1] = (float)(cw * c.w1);
+            wk[2] = (float)(cw * c.w0);
+            wk[3] = (float)(c.s * c.w2);
         }
     }
 

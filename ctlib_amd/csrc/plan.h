@@ -59,6 +59,11 @@ struct HostPlan {
     // {64*lk0*s, lk0*a + lk1, 64*lk2*s, lk2*a + lk3}, so that at pixel edge p the lookup
     // argument is (c0*p/64 + c1) / (c2*p/64 + c3) (the bin-to-pixel kernels' fp32 path)
     std::vector<float> adj_lk;
+    // [V][max_lines][4] the line's pixel weight inv_s / ((w0 + w1 p)^2 + w2) rewritten for fp32:
+    // with c = sqrt(s), {64 c w1, c w1, c w0, c^2 w2}, so that at pixel p = lane + 64 k the
+    // weight is 1 / (u^2 + c^2 w2), u = k * (64 c w1) + (lane * c w1 + c w0) (the weighted
+    // bin-to-pixel kernel's fp32 path; the call's scale multiplies its output)
+    std::vector<float> adj_wk;
     std::vector<int> xviews, yviews;     // view indices per sweep axis, ascending
     std::vector<double> ramp_taps;       // [2D-1] RL taps h(-(D-1)..D-1)
     std::vector<double> ramp_preweight;  // [D] fbp pre-weight (output bin order)

@@ -164,7 +164,10 @@ int ctlib_plan_view_info(const ctlib_plan* plan, int view, int* xmode, int* rev)
  * atan(t) ~ t (c0 + c1 t^2 + ... + c5 t^10) on |t| <= T, then T, then 1 if the
  * kernels use it (else they call atanf), 10 the view lookup composed with each
  * line's pixel-edge map [V][max(H,W)][4] f32 {64 lk0 s, lk0 a + lk1, 64 lk2 s,
- * lk2 a + lk3}.  With dst == NULL only *bytes is set. */
+ * lk2 a + lk3}, 11 each line's pixel weight (1/s) / ((w0 + w1 p)^2 + w2) as
+ * [V][max(H,W)][4] f32 {64 c w1, c w1, c w0, c^2 w2}, c = sqrt(s): the weight at
+ * pixel p = j + 64 k is 1 / (u^2 + c^2 w2) with u = k (64 c w1) + j c w1 + c w0.
+ * With dst == NULL only *bytes is set. */
 int ctlib_plan_export(const ctlib_geom* geom, int table, void* dst, size_t* bytes);
 
 /* Heterogeneous batch: slices of different geometry (image size, views, bins,

@@ -156,6 +156,42 @@ def test_composed_lookup_matches_view_lookup(st):
             np.testing.assert_allclose(got, want, rtol=0, atol=2e-6 * max(1.0, np.max(np.abs(want))))
 
 
+@pytest.mark.parametrize("st", [0, 1, 2])
+def test_prescaled_pixel_weights(st):
+    """Plan table adj_wk - each line's pixel weight rewritten as {64 c w1, c w1, c w0, c^2 w2} with
+    c = sqrt(s), which the fp32 weighted bin->pixel kernel evaluates at pixel p = j + 64 k as
+    1 / (u^2 + c^2 w2), u = k (64 c w1) + j c w1 + c w0 (fp32 FMAs) - equals the weight the oracle
+    and the fp64 path use, (1/s) / ((w0 + w1 p)^2 + w2) from the line's fp64 coefficients, to fp32
+    rounding; for parallel beam it is 1/s; for fan-ED it is 1/(s u^2) with u the distance ratio
+    1 - <source, pixel centre>/R^2 (w2 = 0)."""
+    # 150 x 130 pixels of 0.01: lines over two 64-pixel groups
+    if st == 2:
+        o = phantom.para_options(30, 19, 150, 130, 0.01, 0.13, Ang0=0.25, dAng=2 * math.pi / 30, binshift=-0.03)
+    else:
+        o = phantom.fan_options(30, 19, 150, 130, 0.01, 0.03 if st == 1 else 0.13, 3.0, 2.0, st, Ang0=0.25,
+                                binshift=-0.03)
+    g = abi.parse_options(o)
+    V = g.views
+    nmax = max(g.width, g.height)
+    vp = abi.plan_export(o, "views")
+    lines = abi.plan_export(o, "lines").reshape(V, nmax)
+    wk = abi.plan_export(o, "adj_wk").reshape(V, nmax, 4)
+    for v in range(V):
+        n = int(vp[v]["npix"])
+        p = np.arange(n)
+        j, k = (p % 64).astype(np.float32), (p // 64).astype(np.float32)
+        for l in range(int(vp[v]["nlines"])):
+            c = lines[v, l]
+            want = c["inv_s"] / ((c["w0"] + c["w1"] * p) ** 2 + c["w2"])
+            q = wk[v, l]
+            u0 = (q[1] * j + q[2]).astype(np.float32)
+            u = (k * q[0] + u0).astype(np.float32)
+            got = 1.0 / (u.astype(np.float64) ** 2 + np.float64(q[3]))
+            np.testing.assert_allclose(got, want, rtol=2e-6, atol=0)
+            if st == 2:
+                np.testing.assert_allclose(want, c["inv_s"], rtol=1e-15)
+
+
 @pytest.mark.parametrize("dDet, expect_poly", [(0.03, True), (0.0011, True), (0.0005, True), (0.0016, False)])
 def test_fan_ea_atan_polynomial(dDet, expect_poly):
     """The fan-EA lookup replaces atanf by an odd polynomial on |t| <= T: T must cover every bin edge's
