@@ -697,18 +697,22 @@ __device__ __forceinline__ void fwd_block_w(const DevPlan& P, const T* __restric
     const int hi = n + 1;
     // LineCoef word `lane` (lanes 0..7) of this wave's view, one line ahead
     const double* crow = reinterpret_cast<const double*>(P.lines + (size_t)vw[0] * P.max_lines) + (lane & 7);
-    auto load_line = [&](int l, T (&xv)[PPL], double& cw) {
+    // fp32 data: the line's pre-scaled weight coefficients {., c w1, c w0, c^2 w2} (uniform)
+    const float4* wrow = P.adj_wk + (size_t)vw[0] * P.max_lines;
+    auto load_line = [&](int l, T (&xv)[PPL], double& cw, float4& wk) {
         const T* row = base + (size_t)l * n;
 #pragma unroll
         for (int k = 0; k < PPL; k++) xv[k] = (FULL || p0 + k < n) ? row[p0 + k] : T(0);
         cw = crow[(size_t)l * LC_WORDS];
+        if (sizeof(T) == 4) wk = wrow[l];
     };
     const int lend = le < 0 ? nlines : le;
     T xv[PPL];
     double cw = 0.0;
+    float4 wk = make_float4(0.f, 0.f, 0.f, 1.f);
 #pragma unroll
     for (int k = 0; k < PPL; k++) xv[k] = T(0);
-    if (lb < lend) load_line(lb, xv, cw);
+    if (lb < lend) load_line(lb, xv, cw, wk);
 
     for (int l = lb; l < lend; l++) {
         const double s = readlane_d(cw, LC_S), A = readlane_d(cw, LC_A), Bc = readlane_d(cw, LC_B);
@@ -716,12 +720,12 @@ __device__ __forceinline__ void fwd_block_w(const DevPlan& P, const T* __restric
         const double w2 = readlane_d(cw, LC_W2), is = readlane_d(cw, LC_INVS);
         double val[PPL];
         if constexpr (sizeof(T) == 4) {
-            const float w0f = (float)w0, w1f = (float)w1, w2f = (float)w2, isf = (float)is;
+            // weight (1/s) / ((w0 + w1 p)^2 + w2) = 1 / (u^2 + c^2 w2), u = k c w1 + (p0 c w1 + c w0)
+            const float u0 = __builtin_fmaf(wk.y, (float)p0, wk.z);
 #pragma unroll
             for (int k = 0; k < PPL; k++) {
-                const float u = __builtin_fmaf(w1f, (float)(p0 + k), w0f);
-                const float d = __builtin_fmaf(u, u, w2f);
-                const float v = (float)xv[k] * (isf * __builtin_amdgcn_rcpf(d));
+                const float u = k == 0 ? u0 : __builtin_fmaf((float)k, wk.y, u0);
+                const float v = (float)xv[k] * __builtin_amdgcn_rcpf(__builtin_fmaf(u, u, wk.w));
                 val[k] = (FULL || p0 + k < n) ? (double)v : 0.0;
             }
         } else {
@@ -736,7 +740,7 @@ __device__ __forceinline__ void fwd_block_w(const DevPlan& P, const T* __restric
         double run = 0.0;
 #pragma unroll
         for (int k = 0; k < PPL; k++) run += val[k];
-        if (l + 1 < lend) load_line(l + 1, xv, cw);
+        if (l + 1 < lend) load_line(l + 1, xv, cw, wk);
         // Table of this line: the wave's reads of the previous line's table were
         // consumed above (in-order LDS within a wave), so it can be overwritten.
         const double inc = wave_scan_incl(run, lane);
