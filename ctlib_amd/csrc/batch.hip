@@ -2,9 +2,10 @@
 // count and scan type, run as one persistent fused forward + adjoint launch.
 //
 // Per slice s the work is  y_s = A_s x_s,  grad_s = A_s^T (A_s x_s - m_s),
-// split into work items of four phases:
+// split into work items of five phases:
 //   T  band of 64 image rows -> Y-axis line copy               (slices with Y-axis views)
-//   F  G=4 views of one sweep axis: projection + residual bin tables
+//   L  16 image lines of one sweep axis -> their cumulative line tables
+//   F  G=4 views of one sweep axis: projection over the line tables + residual bin tables
 //   A  tile of lines of one sweep axis: projection_t over all views of the axis
 //   C  band of 64 image rows: X part + transposed Y part       (slices with Y-axis views)
 // The item list is ordered phase by phase (within a phase by the tuned order)
@@ -33,9 +34,10 @@ using namespace dd;
 
 namespace {
 
-enum { IT_T = 0, IT_F = 1, IT_A = 2, IT_C = 3 };
+enum { IT_T = 0, IT_L = 1, IT_F = 2, IT_A = 3, IT_C = 4, NPHASE = 5 };
 constexpr int BAND = 64;   // image rows per T / C item
 constexpr int BG = 4;      // views per F item
+constexpr int LL = 16;     // image lines per L item (4 per wave)
 
 struct SliceDev {
     DevPlan P;
@@ -46,11 +48,13 @@ struct SliceDev {
     void* xT;
     double2* tabs;
     void* yt;
+    double2* lt;  // line tables: X-axis lines then Y-axis lines, ltab_n<M>() entries each
     int fcls;   // forward variant (edge-slot class, see run_fwd)
     int acls;   // adjoint variant: line class * 4 + GEO (GEO_EAP for fan-EA with the polynomial atan)
     int ngx, ngy, ntx, nty, nband;
+    int nlx, nly;  // L items of each axis
     int tileL;  // image lines per adjoint item
-    int pad[2];
+    int pad[3];
 };
 
 struct Item {
@@ -58,33 +62,46 @@ struct Item {
 };
 
 
-// Forward of one F item.  Edge slots per view: 12 (D < 768), 20 (D < 1280) or 24 (D < 1536); 512,
-// 1024 or 1536 bins: 8, 16 or 24 slots and the last edge packed.  Pixels per thread M = 2 (lines
-// <= 512) or, in a batch with a slice beyond 512 x 512 (LARGE), M = 4 for every slice (lines <= 1024).
+// Pixels per thread of the line tables: M = 2 (lines <= 512) or, in a batch with a slice beyond
+// 512 x 512 (LARGE), M = 4 for every slice (lines <= 1024).
+template <bool LARGE>
+constexpr int batch_px() { return LARGE ? 4 : 2; }
+
+// Line tables of one L item: lines idx*LL .. idx*LL + LL - 1 of one axis, four per wave.
+template <typename T, bool LARGE>
+__device__ __forceinline__ void run_lines(const SliceDev& sd, int mode, int idx) {
+    constexpr int M = batch_px<LARGE>();
+    const DevPlan& P = sd.P;
+    const int nl = mode == MODE_Y ? P.W : P.H, n = mode == MODE_Y ? P.H : P.W;
+    const T* src = (const T*)(mode == MODE_Y ? sd.xT : sd.x);
+    double2* lt = sd.lt + (size_t)(mode == MODE_Y && P.nx ? P.H : 0) * ltab_n<M>();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int k = 0; k < LL / 4; k++) {
+        const int l = idx * LL + wave * (LL / 4) + k;
+        if (l < nl) line_table_wave<T, M>(src + (size_t)l * n, n, lt + (size_t)l * ltab_n<M>(), lane);
+    }
+}
+
+// Forward of one F item over the slice's line tables.  Edge slots per view: 12 (D < 768), 20
+// (D < 1280) or 24 (D < 1536); 512, 1024 or 1536 bins: 8, 16 or 24 slots and the last edge packed.
 template <typename T, bool LARGE>
 __device__ __forceinline__ void run_fwd(const SliceDev& sd, int mode, int grp, unsigned char* smem) {
-    const T* src = (const T*)(mode == MODE_Y ? sd.xT : sd.x);
+    constexpr int M = batch_px<LARGE>();
     T* y = (T*)sd.y;
     const T* m = (const T*)sd.m;
-    constexpr int M = LARGE ? 4 : 2;
+    const DevPlan& P = sd.P;
+    const double2* lt = sd.lt + (size_t)(mode == MODE_Y && P.nx ? P.H : 0) * ltab_n<M>();
     switch (sd.fcls) {
-    case 0: fwd_block<T, 1, 12, M, false, true>(sd.P, src, y, 0, mode, grp, sd.P.post_prj, m, sd.tabs, smem); break;
-    case 2:
-        fwd_block<T, 1, 8, M, false, true, false, true>(sd.P, src, y, 0, mode, grp, sd.P.post_prj, m, sd.tabs, smem);
-        break;
-    case 3:
-        fwd_block<T, 1, 16, M, false, true, false, true>(sd.P, src, y, 0, mode, grp, sd.P.post_prj, m, sd.tabs, smem);
-        break;
+    case 0: fwd_block_lt<T, 1, 12, M, true, false, 2>(P, lt, y, 0, mode, grp, P.post_prj, m, sd.tabs, smem, 0, -1, nullptr); break;
+    case 2: fwd_block_lt<T, 1, 8, M, true, true, 2>(P, lt, y, 0, mode, grp, P.post_prj, m, sd.tabs, smem, 0, -1, nullptr); break;
+    case 3: fwd_block_lt<T, 1, 16, M, true, true, 2>(P, lt, y, 0, mode, grp, P.post_prj, m, sd.tabs, smem, 0, -1, nullptr); break;
     case 4:
-        if constexpr (LARGE)
-            fwd_block<T, 1, 24, M, false, true>(sd.P, src, y, 0, mode, grp, sd.P.post_prj, m, sd.tabs, smem);
+        if constexpr (LARGE) fwd_block_lt<T, 1, 24, M, true, false, 2>(P, lt, y, 0, mode, grp, P.post_prj, m, sd.tabs, smem, 0, -1, nullptr);
         break;
     case 5:
-        if constexpr (LARGE)
-            fwd_block<T, 1, 24, M, false, true, false, true>(sd.P, src, y, 0, mode, grp, sd.P.post_prj, m, sd.tabs,
-                                                             smem);
+        if constexpr (LARGE) fwd_block_lt<T, 1, 24, M, true, true, 2>(P, lt, y, 0, mode, grp, P.post_prj, m, sd.tabs, smem, 0, -1, nullptr);
         break;
-    default: fwd_block<T, 1, 20, M, false, true>(sd.P, src, y, 0, mode, grp, sd.P.post_prj, m, sd.tabs, smem); break;
+    default: fwd_block_lt<T, 1, 20, M, true, false, 2>(P, lt, y, 0, mode, grp, P.post_prj, m, sd.tabs, smem, 0, -1, nullptr); break;
     }
 }
 
@@ -137,19 +154,23 @@ __device__ __forceinline__ void run_band(const SliceDev& sd, int type, int band,
 }
 
 // ctr layout: [0] dequeue counter, [1] wait-timeout flag (own 256-byte line), then per-slice done
-// counters of T, F and A.
+// counters of T, L, F and A.
 __global__ void k_batch_reset(unsigned int* ctr, int n) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) ctr[i] = 0u;
 }
 
+// Register budget: 128 VGPRs (four waves per SIMD) for batches of slices within 512^2 and 1279
+// bins - 7.22 ms for the 64-slice batch against 7.66 ms at 168 (three waves) - and 168 for the
+// LARGE variant, whose 24-slot forward needs more.
 template <typename T, bool LARGE>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_batch(const SliceDev* __restrict__ slices, const Item* __restrict__ items,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LARGE ? 3 : 4))) void k_batch(const SliceDev* __restrict__ slices, const Item* __restrict__ items,
                                                int nitems, unsigned int* __restrict__ ctr, int nslices) {
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ int s_item;
     unsigned int* next = ctr;
     unsigned int* tdone = ctr + 64;
-    unsigned int* fdone = tdone + nslices;
+    unsigned int* ldone = tdone + nslices;
+    unsigned int* fdone = ldone + nslices;
     unsigned int* adone = fdone + nslices;
     for (;;) {
         if (threadIdx.x == 0) s_item = (int)atomicAdd(next, 1u);
@@ -162,9 +183,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
         if (threadIdx.x == 0) {
             const unsigned int* c = nullptr;
             unsigned int need = 0;
-            if (I.type == IT_F && I.mode == MODE_Y) {
+            if (I.type == IT_L && I.mode == MODE_Y) {
                 c = tdone + I.slice;
                 need = sd.nband;
+            } else if (I.type == IT_F) {
+                c = ldone + I.slice;
+                need = sd.nlx + sd.nly;
             } else if (I.type == IT_A) {
                 c = fdone + I.slice;
                 need = sd.ngx + sd.ngy;
@@ -192,6 +216,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
         case IT_T:
             run_band<T>(sd, IT_T, I.idx, reinterpret_cast<T(*)[33]>(smem));
             done = tdone + I.slice;
+            break;
+        case IT_L:
+            run_lines<T, LARGE>(sd, I.mode, I.idx);
+            done = ldone + I.slice;
             break;
         case IT_F:
             run_fwd<T, LARGE>(sd, I.mode, I.idx, smem);
@@ -236,7 +264,7 @@ struct ctlib_batch {
     unsigned int* d_ctr = nullptr;
     int nctr = 0;
     size_t ws_bytes = 0;
-    std::vector<size_t> ws_off;    // per slice: xT, tabs, yt offsets
+    std::vector<size_t> ws_off;    // per slice: xT, tabs, yt, line-table offsets
     int blocks_per_cu = 2;
     int order = 0;
     int num_cu = 256;
@@ -259,12 +287,14 @@ size_t esize(int dtype) { return dtype == CTLIB_F64 ? 8 : 4; }
 void build_items(ctlib_batch* b) {
     b->items.clear();
     b->item_cost.clear();
-    std::vector<std::pair<double, Item>> ph[4];
+    std::vector<std::pair<double, Item>> ph[NPHASE];
     for (int s = 0; s < b->nslices; s++) {
         const SliceDev& sd = b->slices[s];
         const DevPlan& P = sd.P;
         if (P.ny)
             for (int i = 0; i < sd.nband; i++) ph[IT_T].push_back({(double)BAND * P.W, Item{IT_T, s, 0, i}});
+        for (int i = 0; i < sd.nlx; i++) ph[IT_L].push_back({(double)LL * P.W, Item{IT_L, s, MODE_X, i}});
+        for (int i = 0; i < sd.nly; i++) ph[IT_L].push_back({(double)LL * P.H, Item{IT_L, s, MODE_Y, i}});
         for (int g = 0; g < sd.ngx; g++)
             ph[IT_F].push_back({(double)P.H * (P.D + 1) * BG, Item{IT_F, s, MODE_X, g}});
         for (int g = 0; g < sd.ngy; g++)
@@ -275,7 +305,7 @@ void build_items(ctlib_batch* b) {
         if (P.ny)
             for (int i = 0; i < sd.nband; i++) ph[IT_C].push_back({(double)BAND * P.W, Item{IT_C, s, 0, i}});
     }
-    for (int p = 0; p < 4; p++) {
+    for (int p = 0; p < NPHASE; p++) {
         if (b->order == 0) {  // longest item first within the phase
             std::stable_sort(ph[p].begin(), ph[p].end(),
                              [](const std::pair<double, Item>& a, const std::pair<double, Item>& c) {
@@ -313,6 +343,7 @@ int ctlib_batch_create(const ctlib_geom* geoms, int nslices, int device, int dty
         return bfail(code, why);
     };
     size_t off = 0;
+    std::vector<size_t> lines_total;
     for (int s = 0; s < nslices; s++) {
         const ctlib_geom& g = geoms[s];
         if (g.width > 1024 || g.height > 1024 || g.dets > 1536)
@@ -347,6 +378,8 @@ int ctlib_batch_create(const ctlib_geom* geoms, int nslices, int device, int dty
         sd.ntx = P.nx ? (P.H + L - 1) / L : 0;
         sd.nty = P.ny ? (P.W + L - 1) / L : 0;
         sd.nband = (P.H + BAND - 1) / BAND;
+        sd.nlx = P.nx ? (P.H + LL - 1) / LL : 0;
+        sd.nly = P.ny ? (P.W + LL - 1) / LL : 0;
         b->slices.push_back(sd);
         const size_t es = esize(dtype);
         b->ws_off.push_back(off);
@@ -355,8 +388,18 @@ int ctlib_batch_create(const ctlib_geom* geoms, int nslices, int device, int dty
         off += align_up((size_t)P.V * (P.D + 2) * sizeof(double2));  // tabs
         b->ws_off.push_back(off);
         off += align_up((size_t)P.H * P.W * es);                   // yt
-        b->lds = std::max(b->lds, fwd_block_lds(nmax, BG, false, 16));
+        b->ws_off.push_back(0);  // line tables: placed once every slice's size class is known
+        lines_total.push_back((size_t)(P.nx ? P.H : 0) + (P.ny ? P.W : 0));
         b->lds = std::max(b->lds, adj_block_lds(P.D, L));
+    }
+    // line tables last (their entry count per line depends on the batch's pixels-per-thread class)
+    {
+        const size_t tabn = b->large ? ltab_n<batch_px<true>()>() : ltab_n<batch_px<false>()>();
+        for (int s = 0; s < nslices; s++) {
+            b->ws_off[4 * s + 3] = off;
+            off += align_up(lines_total[s] * tabn * sizeof(double2));
+        }
+        b->lds = std::max(b->lds, b->large ? fwd_lt_lds<batch_px<true>()>(BG, 2) : fwd_lt_lds<batch_px<false>()>(BG, 2));
     }
     b->lds = std::max(b->lds, (size_t)32 * 33 * 8);
     b->ws_bytes = off;
@@ -364,7 +407,7 @@ int ctlib_batch_create(const ctlib_geom* geoms, int nslices, int device, int dty
     if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
         b->num_cu = prop.multiProcessorCount;
     build_items(b);
-    b->nctr = 64 + 3 * nslices;
+    b->nctr = 64 + 4 * nslices;
     if (hipMalloc(&b->d_slices, sizeof(SliceDev) * nslices) != hipSuccess ||
         hipMalloc(&b->d_items, sizeof(Item) * b->items.size()) != hipSuccess ||
         hipMalloc(&b->d_ctr, sizeof(unsigned int) * b->nctr) != hipSuccess)
@@ -420,9 +463,10 @@ int ctlib_batch_bind(ctlib_batch* b, const void* const* x, const void* const* m,
         sd.m = m[s];
         sd.y = y ? y[s] : nullptr;
         sd.grad = grad[s];
-        sd.xT = w + b->ws_off[3 * s];
-        sd.tabs = reinterpret_cast<double2*>(w + b->ws_off[3 * s + 1]);
-        sd.yt = w + b->ws_off[3 * s + 2];
+        sd.xT = w + b->ws_off[4 * s];
+        sd.tabs = reinterpret_cast<double2*>(w + b->ws_off[4 * s + 1]);
+        sd.yt = w + b->ws_off[4 * s + 2];
+        sd.lt = reinterpret_cast<double2*>(w + b->ws_off[4 * s + 3]);
     }
     int prev = -1;
     (void)hipGetDevice(&prev);
