@@ -174,8 +174,8 @@ int ctlib_plan_export(const ctlib_geom* geom, int table, void* dst, size_t* byte
  * scan_type), each with one image x_s and one measured sinogram m_s, run as
  * ONE persistent fused launch computing y_s = A_s x_s and
  * grad_s = A_s^T (A_s x_s - m_s) for every slice.  Work items (line copies,
- * projection groups, projection_t tiles, axis combines) are dequeued from one
- * atomic counter; per-slice counters order the phases.  ctlib_batch_run
+ * line tables, projection groups, projection_t tiles, axis combines) are dequeued
+ * from one atomic counter; per-slice counters order the phases.  ctlib_batch_run
  * allocates nothing, synchronises nothing and records no memset, so it may be
  * captured in a HIP graph.  Slices: width, height <= 1024, dets <= 1536 (a batch
  * with any slice beyond 512 x 512 or 1279 bins runs a kernel variant whose
