@@ -334,29 +334,25 @@ __host__ __device__ inline size_t fwd_block_lds(int nmax, int G, bool weighted, 
 template <int M>
 __host__ __device__ constexpr int ltab_n() { return NT * M + 2; }
 
-// One line's table by one wave: lane L owns pixels 4M*L .. 4M*L + 4M - 1 (n <= 256*M);
-// the entries are those line_tables writes for the same line.
+// One line's table by one wave (n <= 256*M): in round k lane L takes pixel p = 64 k + L, a wave
+// scan gives the pixels before it, and the 64 lanes store 64 consecutive entries (1 KB, fully
+// coalesced); the entries are those line_tables writes for the same line.
 template <typename T, int M>
 __device__ __forceinline__ void line_table_wave(const T* __restrict__ row, int n, double2* __restrict__ out, int lane) {
-    constexpr int PPL = 4 * M;
-    const int p0 = lane * PPL;
-    double v[PPL];
-    double run = 0.0;
+    double carry = 0.0;  // sum of the pixels of the earlier rounds
 #pragma unroll
-    for (int i = 0; i < PPL; i++) {
-        v[i] = p0 + i < n ? (double)row[p0 + i] : 0.0;
-        run += v[i];
+    for (int k = 0; k < 4 * M; k++) {
+        if (64 * k >= n) break;  // wave-uniform
+        const int p = 64 * k + lane;
+        const double v = p < n ? (double)row[p] : 0.0;
+        const double inc = wave_scan_incl(v, lane);
+        if (p < n) out[p + 1] = make_double2(carry + (inc - v) - (double)(p + 1) * v, v);
+        carry += readlane_d(inc, 63);
     }
-    const double inc = wave_scan_incl(run, lane);
-    double pref = inc - run;
-#pragma unroll
-    for (int i = 0; i < PPL; i++) {
-        const int p = p0 + i;
-        if (p < n) out[p + 1] = make_double2(pref - (double)(p + 1) * v[i], v[i]);
-        pref += v[i];
+    if (lane == 0) {
+        out[0] = make_double2(0.0, 0.0);
+        out[n + 1] = make_double2(carry, 0.0);
     }
-    if (lane == 0) out[0] = make_double2(0.0, 0.0);
-    if (lane == 63) out[n + 1] = make_double2(inc, 0.0);
 }
 
 // One line -> bin work item: batch element b, sweep axis `mode`, views
