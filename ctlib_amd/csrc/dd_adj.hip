@@ -96,6 +96,11 @@ __global__ __launch_bounds__(256) void k_bprj_sv(DevPlan P, const double2* __res
         // pixel weight in fp32, as in the bin->pixel kernels; the fp64 edge differences are rounded to
         // fp32 before the weight is applied.
         const float4* alk = P.adj_lk + (size_t)v * P.max_lines;
+        // pixel weights from the plan's pre-scaled coefficients {64 c w1, c w1, c w0, c^2 w2}
+        // (weight inv_s / ((w0 + w1 p)^2 + w2) = 1 / (u^2 + c^2 w2), u = c (w0 + w1 p)), with the
+        // call's scale folded in per line: u / sqrt(wscale) and c^2 w2 / wscale
+        const float4* awk = P.adj_wk + (size_t)v * P.max_lines;
+        const float rs = (float)(1.0 / sqrt(wscale)), rs2 = (float)(1.0 / wscale);  // wscale > 0
         if (vp.xmode) {
             for (int r = wave; r < H; r += NT / 64) {
                 const LineCoef c = lc[r];
@@ -108,8 +113,9 @@ __global__ __launch_bounds__(256) void k_bprj_sv(DevPlan P, const double2* __res
                     if (i > 0) pir += lk.s64;
                     g[i] = eval_G_lk<GEO>(t, pir, lk, i, lu, hi);
                 }
-                const float w1f = (float)c.w1, w2f = (float)c.w2, wsf = (float)(c.inv_s * wscale);
-                const float u0 = __builtin_fmaf(w1f, (float)lane, (float)c.w0), w1x64 = 64.0f * w1f;
+                const float4 wk = awk[r];
+                const float w1x64 = wk.x * rs, w1s = wk.y * rs, w2s = wk.w * rs2;
+                const float u0 = __builtin_fmaf(w1s, (float)lane, wk.z * rs);
                 // right edge of pixel lane + 64 i: edge lane + 1 of group i, for lane 63 edge 0 of
                 // group i + 1 - both from the groups' rotations by one lane
                 const bool last = lane == 63;
@@ -125,7 +131,7 @@ __global__ __launch_bounds__(256) void k_bprj_sv(DevPlan P, const double2* __res
                     }
                     const int p = lane + 64 * i;
                     const float u = i == 0 ? u0 : __builtin_fmaf((float)i, w1x64, u0);
-                    const float w = wsf * __builtin_amdgcn_rcpf(__builtin_fmaf(u, u, w2f));
+                    const float w = __builtin_amdgcn_rcpf(__builtin_fmaf(u, u, w2s));
                     if (p < W) dst[p] = (float)(right - g[i]) * w;
                 }
             }
@@ -138,8 +144,8 @@ __global__ __launch_bounds__(256) void k_bprj_sv(DevPlan P, const double2* __res
                 // edge p of the column: axis coordinate a + s p (stepped in fp64), lookup argument
                 // (c0 p + c1) / (c2 p + c3) with p held exactly as a float counter
                 const float cx = ck.x * (1.0f / 64.0f), cz = ck.z * (1.0f / 64.0f);
-                const float w0f = (float)c.w0, w1f = (float)c.w1, w2f = (float)c.w2;
-                const float wsf = (float)(c.inv_s * wscale);
+                const float4 wk = awk[ok ? col : W - 1];
+                const float w0s = wk.z * rs, w1s = wk.y * rs, w2s = wk.w * rs2;
                 auto G_edge = [&](double pi, float pf) {
                     const float num = __builtin_fmaf(cx, pf, ck.y);
                     float q;
@@ -170,8 +176,8 @@ __global__ __launch_bounds__(256) void k_bprj_sv(DevPlan P, const double2* __res
                     pi += c.s;
                     pf += 1.0f;
                     const double gi = G_edge(pi, pf);
-                    const float u = __builtin_fmaf(w1f, pix, w0f);
-                    const float w = wsf * __builtin_amdgcn_rcpf(__builtin_fmaf(u, u, w2f));
+                    const float u = __builtin_fmaf(w1s, pix, w0s);
+                    const float w = __builtin_amdgcn_rcpf(__builtin_fmaf(u, u, w2s));
                     if (ok) dst[-(ptrdiff_t)i * W] = (float)(gi - gprev) * w;
                     gprev = gi;
                 }
@@ -356,6 +362,7 @@ hipError_t sv_geo(const DevPlan& P, const double2* tabs, void* out, int B, doubl
 hipError_t launch_bprj_sv(const DevPlan& P, int dtype, const double2* tabs, void* out, int B, double wscale,
                           hipStream_t s) {
     if (B <= 0) return hipSuccess;
+    if (!(wscale > 0.0)) return hipErrorInvalidValue;  // the fp32 path folds 1/sqrt(wscale) into its weights
     if ((size_t)B * P.V > (size_t)INT32_MAX || (size_t)(P.D + 2) * sizeof(double2) > 64 * 1024)
         return hipErrorInvalidValue;
     if (dtype == DT_F32) return sv_geo<float>(P, tabs, out, B, wscale, s);

@@ -103,7 +103,7 @@ hipError_t launch_combine_parts(void* out, const void* xs, const void* ys, int d
                                 bool hasX, bool hasY, hipStream_t s);
 
 // Per-view weighted backprojection (backprojection_sv): out[b][v][H][W] = view v's term of
-// backprojection, pixel weight * wscale, from the bin tables of launch_bin_tables.
+// backprojection, pixel weight * wscale (wscale > 0), from the bin tables of launch_bin_tables.
 hipError_t launch_bprj_sv(const DevPlan& P, int dtype, const double2* tabs, void* out, int B, double wscale,
                           hipStream_t s);
 
