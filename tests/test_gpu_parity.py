@@ -175,6 +175,33 @@ def test_square_line_lengths(dev, n, st, op):
     assert rel_l2(got, ref) <= F32_TOL, (op, st, n, rel_l2(got, ref))
 
 
+def test_line_tables_setting(dev):
+    """ctlib_plan_set_line_tables: 0 (prebuilt tables, default) or 1 (built per block); anything else
+    is refused.  The prebuilt tables live in the workspace: with the same launch split, projection's
+    workspace grows by B * (H + W) * (256 M + 2) * 16 bytes (M = 2 for 512-pixel lines), and the
+    per-block setting needs none of it."""
+    from ctlib_amd import _C
+    o = phantom.CONFIGS[1]["options"]
+    ot = torch.from_numpy(o)
+    d = dev.index or 0
+    g = O.parse(o)
+    B = 4
+    with pytest.raises(Exception):
+        _C.set_plan_line_tables(ot, d, 2)
+    try:
+        _C.set_plan_chunks(ot, d, 1, 1)  # same (unsplit) launches for both settings
+        _C.set_plan_line_tables(ot, d, 1)
+        per_block = _C.workspace_bytes(ot, _C.OP_PRJ, B, False, d)
+        _C.set_plan_line_tables(ot, d, 0)
+        prebuilt = _C.workspace_bytes(ot, _C.OP_PRJ, B, False, d)
+    finally:
+        _C.set_plan_line_tables(ot, d, 0)
+        _C.set_plan_chunks(ot, d, 0, 0)
+    tables = B * (g["width"] + g["height"]) * (256 * 2 + 2) * 16
+    assert prebuilt - per_block >= tables
+    assert prebuilt - per_block < tables + 4096
+
+
 @pytest.mark.parametrize("per_block", [0, 1])
 @pytest.mark.parametrize("wh", [(256, 256), (300, 40), (40, 640), (1024, 64), (1800, 48), (48, 2048)])
 @pytest.mark.parametrize("st", [0, 2])

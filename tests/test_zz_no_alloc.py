@@ -92,6 +92,33 @@ def test_noalloc_matches_run(dev, cfg, dtype):
         plan.close()
 
 
+def test_noalloc_builds_line_tables_per_block_when_they_do_not_fit(dev):
+    """projection at fan-ED 512^2, 736 x 736, B=8: its prebuilt line tables alone (8 x 1024 lines x
+    514 entries x 16 bytes = 67 MB) exceed the fixed array, so ctlib_run_noalloc builds the tables in
+    each block (and splits the lines into fewer chunks) - still ctlib_run's result, which comes from
+    the prebuilt tables, to fp32 summation order."""
+    opts = phantom.CONFIGS[1]["options"]
+    plan = Plan(opts, dev.index or 0)
+    try:
+        g = plan.g
+        B = 8
+        assert B * (g.width + g.height) * 514 * 16 > SCRATCH_BYTES
+        assert plan.lib.ctlib_workspace_bytes(plan.p, 0, B, 0) > SCRATCH_BYTES
+        gen = torch.Generator().manual_seed(7)
+        for Bc in (B, 3):  # the second call needs no fallback
+            x = torch.rand(Bc, 1, g.height, g.width, generator=gen).to(dev)
+            ref = torch.empty(Bc, 1, g.views, g.dets, device=dev)
+            got = torch.full((Bc, 1, g.views, g.dets), float("nan"), device=dev)
+            print(f"noalloc projection per-block tables B={Bc}", flush=True)
+            plan.run(0, x, ref)
+            rc = plan.run_noalloc(0, x, got)
+            assert rc == 0, plan.lib.ctlib_last_error()
+            torch.cuda.synchronize()
+            assert rel(got, ref) <= 1e-6, (Bc, rel(got, ref))
+    finally:
+        plan.close()
+
+
 def test_noalloc_refuses_oversized_scratch(dev):
     """fbp at fan-ED 512^2, 736 x 736, B=16 needs the ramp rows and bin tables (> 64 MiB even
     unsplit): refused with the workspace status, the output untouched."""
